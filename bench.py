@@ -1,0 +1,201 @@
+#!/usr/bin/env python3
+"""Throughput of the Toeplitz-hashing hot path on one MI355X.
+
+Metric: amplified key Gbit/s at 2^27-bit blocks (Gbit/s of input key, the reference's own
+convention, PrivacyAmplification.cu:1213), bit-exact.  One step = one block of n = 2^27 bits,
+m = 3n/8, XOR and compression on, with a fresh (dynamic) Toeplitz seed per block -- the
+reference's secure default (config.yaml:17) -- on synthetic Bernoulli(0.5) inputs already
+resident in HBM.  Prints one JSON line.
+
+  python bench.py [--steps K] [--warmup W] [--mode dynamic|static] [--log2n 27]
+                  [--no-cpu-baseline] [--gpus 1]
+"""
+import argparse
+import hashlib
+import json
+import os
+import pathlib
+import sys
+import time
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+
+
+def algorithmic_bytes(n, m, mode):
+    """Compulsory HBM bytes per block of the fused three-pass design (two passes per transform over
+    the n/2-complex FP32 array; unpack, product and epilogue fused; static seeds read a resident
+    spectrum)."""
+    if n <= (1 << 14):
+        return n / 4 + m / 4
+    if mode == "static":
+        return 20 * n + n / 8 + m / 4
+    return 24 * n + n / 4 + m / 4
+
+
+def kernel_bytes(n, m):
+    """Algorithmic bytes per launch of each kernel as built."""
+    return {
+        "k_transpose_bits(key)": n / 8 + n / 8,
+        "k_transpose_bits(seed)": n / 8 + n / 8,
+        "k_fwd_cols(key)": n / 8 + 4 * n,
+        "k_fwd_cols(seed)": n / 8 + 4 * n,
+        "k_rows(seedprep)": 8 * n,
+        "k_rows": 12 * n,
+        "k_inv_cols": 4 * n + m / 4,
+        "k_pack": m / 4 + m / 8 + m / 8,
+        "k_zero_u32": 8,
+    }
+
+
+def cpu_baseline(n, m, budget_s=30.0):
+    """The reference's FP32 method restated on the host cores (oracle/, multithreaded), timed on one
+    bounded sample: one block of the same workload."""
+    from oracle import oracle as O
+    O.build()
+    threads = O.get_threads()
+    key = O.synth_bits(0x1000, n + 1)
+    seed = O.synth_bits(0x2000, n)
+    t0 = time.perf_counter()
+    O.amp_out(key, seed, n, m, method=O.FFT32)
+    dt = time.perf_counter() - t0
+    return {"value": round(n / dt / 1e9, 6), "unit": "Gbit/s", "cores": threads, "kind": "port",
+            "sample": f"1 block n=2^{n.bit_length()-1} m=3n/8 dynamic seed, FP32 FFT restatement of the "
+                      f"reference pipeline (oracle/pa_oracle.c, OpenMP), {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--log2n", type=int, default=27)
+    ap.add_argument("--mode", choices=("dynamic", "static"), default="dynamic")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (blocks are independent: "
+              "N GPUs would run N independent engines with no communication)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import privacyamplification_amd as P
+
+    assert torch.cuda.is_available(), "no GPU visible"
+    dev = torch.device("cuda:0")
+    n = 1 << args.log2n
+    m = 3 * n // 8
+    flags = P.DEFAULT_FLAGS
+    nblk = 2  # distinct resident inputs, alternated so consecutive blocks differ
+    keys, seeds = [], []
+    for i in range(nblk):
+        keys.append(torch.from_numpy(P.synth_bits(0x1000 + i, n + 1).view(np.int32)).to(dev))
+        seeds.append(torch.from_numpy(P.synth_bits(0x2000 + i, n).view(np.int32)).to(dev))
+    out = torch.empty(m // 8 + 16, dtype=torch.uint8, device=dev)
+    margin = torch.zeros(1, dtype=torch.float32, device=dev)
+    eng = P.Engine(device=0)
+    stream = torch.cuda.current_stream()
+    sptr = stream.cuda_stream
+    sids = []
+    if args.mode == "static":
+        sids = [eng.seed_put(P.synth_bits(0x5EED, n), n, m)]
+
+    def step(i, mg=None):
+        j = i % nblk
+        if args.mode == "static":
+            eng.block_device_seeded(keys[j].data_ptr(), sids[0], n, m, flags, out.data_ptr(), sptr,
+                                    mg.data_ptr() if mg is not None else None)
+        else:
+            eng.block_device(keys[j].data_ptr(), seeds[j].data_ptr(), n, m, flags, out.data_ptr(), sptr,
+                             mg.data_ptr() if mg is not None else None)
+
+    for i in range(args.warmup):
+        step(i)
+    torch.cuda.synchronize()
+
+    # correctness of the measured configuration against the committed oracle digests
+    verified = None
+    man = {b["name"]: b for b in json.loads((ROOT / "tests/golden/manifest.json").read_text())["blocks"]}
+    ref_name = ("c2_static_0" if args.mode == "static" else "c2_dynamic_0") if args.log2n == 27 else None
+    step(0, margin)
+    torch.cuda.synchronize()
+    if ref_name:
+        verified = hashlib.sha3_256(out[: m // 8].cpu().numpy().tobytes()).hexdigest() == man[ref_name]["sha3"]
+    max_round_dist = float(margin.item())
+
+    # timed region: K blocks, device events on the launch stream
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    evs[0].record(stream)
+    for i in range(args.steps):
+        step(i)
+        evs[i + 1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    per = [evs[i].elapsed_time(evs[i + 1]) for i in range(args.steps)]
+    total_ms = evs[0].elapsed_time(evs[-1])
+    ms_per_step = total_ms / args.steps
+    med = float(np.median(per))
+
+    # per-kernel durations (HIP events around each launch, separate pass)
+    eng.stage_timing(True)
+    for i in range(args.steps):
+        step(i)
+    st = eng.stage_times()
+    eng.stage_timing(False)
+    kb = kernel_bytes(n, m)
+    kernels = []
+    for name, (ms, cnt) in st.items():
+        avg = ms / cnt
+        b = kb.get(name)
+        kernels.append({"name": name, "avg_ms": round(avg, 5), "launches": cnt,
+                        "alg_bytes": b, "GBs": round(b / avg / 1e6, 1) if b else None})
+    kernels.sort(key=lambda k: -k["avg_ms"])
+    dominant = kernels[0] if kernels else None
+
+    bytes_block = algorithmic_bytes(n, m, args.mode)
+    achieved = bytes_block / (med * 1e-3) / 1e9
+    traffic = None
+    tfile = ROOT / "profiles" / f"traffic_n{args.log2n}_{args.mode}.json"
+    if tfile.exists():
+        traffic = json.loads(tfile.read_text()).get("bytes_per_block")
+
+    result = {
+        "metric": "amplified key Gbit/s at 2^27-bit blocks, bit-exact",
+        "value": round(n * args.steps / (total_ms * 1e-3) / 1e9, 3),
+        "unit": "Gbit/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"single block n=2^{args.log2n} bits, m=3n/8, {args.mode} seed, "
+                               f"XOR key rest + compress, Bernoulli(0.5) key and seed",
+                   "n": n, "m": m, "seed_mode": args.mode, "blocks_per_step": 1},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "scope": "whole block (all launches): algorithmic bytes per block / median block time",
+                     "alg_bytes_per_block": bytes_block, "median_block_ms": round(med, 4)},
+        "kernels": kernels,
+        "dominant_kernel": dominant["name"] if dominant else None,
+        "output_rate_gbit_s": round(m * args.steps / (total_ms * 1e-3) / 1e9, 3),
+        "verified_sha3": verified,
+        "max_round_dist": max_round_dist,
+        "wall_s": round(wall, 3),
+    }
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(n, m)
+    print(json.dumps(result), flush=True)
+    eng.close()
+
+
+if __name__ == "__main__":
+    main()

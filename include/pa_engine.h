@@ -1,0 +1,135 @@
+/*
+ * pa_engine.h -- C ABI of the MI355X Toeplitz-hashing engine (privacy amplification).
+ *
+ * Drop-in for the Toeplitz-hashing stage of nicoboss/PrivacyAmplification.  One call computes,
+ * for a key block of n+1 bits and an n-bit Toeplitz seed, the m-bit
+ *
+ *     amp_out[p] = XOR_{j=0..h} key[j] * seed[(p - j) mod n]   XOR   key[h + 1 + p],   h = n - m,
+ *
+ * bit-exactly as the reference's mainloop does (PrivacyAmplification.cu:1819-2042), on gfx950
+ * through hand-written HIP kernels.  All pointers are plain host or device pointers; no C++ or
+ * torch types cross this boundary.  Every entry point returns PA_OK (0) or a negative pa_status.
+ *
+ * Bit layouts are the reference's wire formats:
+ *   key / seed (input) : RefBits -- little-endian uint32 words, bit p of the stream is
+ *                        (w[p>>5] >> (31-(p&31))) & 1   (PrivacyAmplification.cu:584-593).
+ *   amp_out (output)   : OutBytes -- byte p>>3, bit 7-(p&7), the AMPOUT_REVERSE_ENDIAN=TRUE layout
+ *                        (PrivacyAmplification.h:38, .cu:260-267); with out_reverse_endian=0 the
+ *                        output is RefBits words instead.
+ * Key buffer length: n/32+1 words with PA_XOR_KEY_REST (the reference's key_blocks,
+ * PrivacyAmplification.cu:1428), h/32+1 words without it (reciveDataKey's NoXOR frame,
+ * PrivacyAmplification.cu:1068-1072).  Seed: n/32 words (desired_block).
+ * Output length: m/8 bytes with PA_COMPRESS, n/16 bytes without (sendData,
+ * PrivacyAmplification.cu:1186).
+ */
+#ifndef PA_ENGINE_H
+#define PA_ENGINE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pa_engine pa_engine;
+typedef uint64_t pa_seed_id;
+typedef uint64_t pa_ticket;
+
+/* Block flags: the two 1-byte frames the key server sends ahead of each key
+ * (do_xor_key_rest, do_compress; PrivacyAmplification.cu:1055-1056). */
+enum pa_flags {
+    PA_XOR_KEY_REST = 1u,
+    PA_COMPRESS = 2u,
+    PA_DEFAULT_FLAGS = 3u
+};
+
+enum pa_status {
+    PA_OK = 0,
+    PA_ERR_ARG = -1,         /* null pointer, bad flag, unknown handle */
+    PA_ERR_SIZE = -2,        /* n / m outside the supported family (the reference exits 400) */
+    PA_ERR_DEVICE = -3,      /* HIP runtime error */
+    PA_ERR_NOMEM = -4,       /* device allocation failed or HBM budget exceeded */
+    PA_ERR_TIMEOUT = -5,     /* pa_wait timed out */
+    PA_ERR_NOT_FOUND = -6,   /* seed id / ticket unknown */
+    PA_ERR_UNSUPPORTED = -7  /* valid request this build does not serve */
+};
+
+/* Engine configuration.  The fields mirror config.yaml (PrivacyAmplification/config.yaml:6-124,
+ * defaults PrivacyAmplification.cu:1244-1271) where a field exists there. */
+typedef struct {
+    int32_t device;                 /* gpu_device_id_to_use */
+    uint32_t max_n_log2;            /* largest factor_exp accepted (the reference's maximum is 27) */
+    uint32_t reduction_exp;         /* reduction_exp (config.yaml:42); scaling only, parity-neutral */
+    uint32_t pre_mul_reduction_exp; /* pre_mul_reduction_exp (config.yaml:47); idem */
+    uint64_t hbm_budget_bytes;      /* admission budget for resident seeds + in-flight blocks */
+    uint32_t out_reverse_endian;    /* AMPOUT_REVERSE_ENDIAN (PrivacyAmplification.h:38), default 1 */
+    uint32_t reserved;
+} pa_config;
+
+/* Fills the defaults: device 0, max_n_log2 27, reduction 11 / 5, budget 64 GiB, reverse endian. */
+int pa_config_default(pa_config* cfg);
+
+/* Creates an engine on cfg->device (main's device set-up and buffer allocation,
+ * PrivacyAmplification.cu:1440-1620).  Device buffers are allocated lazily per block size. */
+int pa_create(const pa_config* cfg, pa_engine** out);
+int pa_destroy(pa_engine* eng);
+
+/* Registers a resident Toeplitz seed: its spectrum is computed once and kept in HBM, the
+ * MI355X form of a reused seed (reuse_seed_amount != 0, PrivacyAmplification.cu:1781-1791,
+ * 1898-1910).  seed: n/32 words of host memory.  reuse is kept as metadata (-1 forever, 0 one
+ * block, k>0 k blocks); the caller drops the seed with pa_seed_drop.  m fixes the block shape
+ * the spectrum serves (the key split depends on it only for non-power-of-two n). */
+int pa_seed_put(pa_engine* eng, const uint32_t* seed, uint32_t n, uint32_t m, int32_t reuse,
+                pa_seed_id* id);
+int pa_seed_drop(pa_engine* eng, pa_seed_id id);
+
+/* One block, synchronous, host buffers (the body of one mainloop iteration together with the
+ * receive-side key split and the send-side output, PrivacyAmplification.cu:861-892, 1819-2042,
+ * 1143-1225).  seed: n/32 words (dynamic seed). */
+int pa_run_block(pa_engine* eng, const uint32_t* key, const uint32_t* seed, uint32_t n,
+                 uint32_t m, uint32_t flags, uint8_t* out);
+
+/* Same with a resident seed (static-seed mode). */
+int pa_run_block_seeded(pa_engine* eng, const uint32_t* key, pa_seed_id seed, uint32_t n,
+                        uint32_t m, uint32_t flags, uint8_t* out);
+
+/* Asynchronous form: enqueues one block on the engine's stream and returns a ticket; the host
+ * buffers must stay valid until pa_wait returns for that ticket.  seed_id == 0 with seed != NULL
+ * runs a dynamic seed; seed == NULL uses the resident seed seed_id. */
+int pa_submit(pa_engine* eng, const uint32_t* key, const uint32_t* seed, pa_seed_id seed_id,
+              uint32_t n, uint32_t m, uint32_t flags, uint8_t* out, pa_ticket* ticket);
+/* timeout_ns < 0 waits forever. */
+int pa_wait(pa_engine* eng, pa_ticket ticket, int64_t timeout_ns);
+
+/* Device-resident form: key, seed and out are device pointers, the work is stream-ordered on
+ * `stream` (a hipStream_t, NULL = the engine's stream) and nothing is synchronised.  The engine's
+ * per-size workspace is shared, so calls on one engine must not overlap on different streams.
+ * margin_out (device float, may be NULL) receives max |v - rint(v)| over the block's pre-rounding
+ * values (atomic max; the caller zeroes it). */
+int pa_block_device(pa_engine* eng, const uint32_t* d_key, const uint32_t* d_seed, uint32_t n,
+                    uint32_t m, uint32_t flags, uint8_t* d_out, void* stream, float* d_margin);
+int pa_block_device_seeded(pa_engine* eng, const uint32_t* d_key, pa_seed_id seed, uint32_t n,
+                           uint32_t m, uint32_t flags, uint8_t* d_out, void* stream,
+                           float* d_margin);
+
+/* Per-stage device timing, the counterpart of the reference's STOPWATCH stage timers
+ * (PrivacyAmplification.cu:146-166, 2053-2074).  When enabled, the engine records HIP events
+ * around every kernel it launches (on the launch stream).  pa_stage_times synchronises, then
+ * returns per stage: its name (static string), the summed milliseconds and the launch count
+ * since the last read, and resets the sums.  *n_stages is in: capacity, out: stages written. */
+int pa_stage_timing(pa_engine* eng, int enable);
+int pa_stage_times(pa_engine* eng, const char** names, double* ms, uint32_t* counts,
+                   uint32_t* n_stages);
+
+/* Bytes of device workspace a block of this shape uses (0 for an unsupported shape). */
+uint64_t pa_workspace_bytes(uint32_t n, uint32_t m, uint32_t flags);
+
+/* Synthetic RefBits: xoshiro256** seeded through splitmix64, Bernoulli(p) bits; p == 0.5 takes
+ * two words per 64-bit draw (high half first).  Host memory, nwords = ceil(nbits/32). */
+int pa_synth_bits(uint64_t seed, double p, uint64_t nbits, uint32_t* out);
+
+const char* pa_strerror(int status);
+const char* pa_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

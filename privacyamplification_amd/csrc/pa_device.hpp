@@ -1,0 +1,208 @@
+// pa_device.hpp -- device building blocks of the gfx950 Toeplitz-hashing engine.
+//
+// Complex arithmetic, in-register DFTs of size 2..32, and an in-place Stockham FFT over LDS.
+// All transforms are forward (w = exp(-2 pi i / L)) and unnormalized; inverses are taken as
+// conj(FFT(conj(x))) by the callers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pa {
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// a * (-i)
+__device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }
+
+// exp(-2 pi i k / 32), k in [0, 16): exact constants rounded to float.
+__host__ __device__ constexpr float w32_re(int k) {
+    return k == 0 ? 1.0f : k == 1 ? 9.807852804e-01f : k == 2 ? 9.238795325e-01f
+         : k == 3 ? 8.314696123e-01f : k == 4 ? 7.071067812e-01f : k == 5 ? 5.555702330e-01f
+         : k == 6 ? 3.826834324e-01f : k == 7 ? 1.950903220e-01f : k == 8 ? 0.0f
+         : k == 9 ? -1.950903220e-01f : k == 10 ? -3.826834324e-01f : k == 11 ? -5.555702330e-01f
+         : k == 12 ? -7.071067812e-01f : k == 13 ? -8.314696123e-01f : k == 14 ? -9.238795325e-01f
+         : -9.807852804e-01f;
+}
+__host__ __device__ constexpr float w32_im(int k) {
+    return k == 0 ? 0.0f : k == 1 ? -1.950903220e-01f : k == 2 ? -3.826834324e-01f
+         : k == 3 ? -5.555702330e-01f : k == 4 ? -7.071067812e-01f : k == 5 ? -8.314696123e-01f
+         : k == 6 ? -9.238795325e-01f : k == 7 ? -9.807852804e-01f : k == 8 ? -1.0f
+         : k == 9 ? -9.807852804e-01f : k == 10 ? -9.238795325e-01f : k == 11 ? -8.314696123e-01f
+         : k == 12 ? -7.071067812e-01f : k == 13 ? -5.555702330e-01f : k == 14 ? -3.826834324e-01f
+         : -1.950903220e-01f;
+}
+
+// v * exp(-2 pi i k / R) for a compile-time-foldable k in [0, R/2), R <= 32.
+template <int R>
+__device__ __forceinline__ float2 twc(float2 v, int k) {
+    const int k32 = k * (32 / R);
+    if (k32 == 0) return v;
+    if (k32 == 8) return mul_mi(v);
+    return cmul(v, make_float2(w32_re(k32), w32_im(k32)));
+}
+
+// In-register forward DFT, natural-order input and output (radix-2 decimation in time).
+template <int R>
+__device__ __forceinline__ void dft(float2 (&x)[R]) {
+    if constexpr (R == 2) {
+        const float2 a = x[0], b = x[1];
+        x[0] = cadd(a, b);
+        x[1] = csub(a, b);
+    } else if constexpr (R == 4) {
+        const float2 t0 = cadd(x[0], x[2]), t1 = csub(x[0], x[2]);
+        const float2 t2 = cadd(x[1], x[3]), t3 = mul_mi(csub(x[1], x[3]));
+        x[0] = cadd(t0, t2);
+        x[2] = csub(t0, t2);
+        x[1] = cadd(t1, t3);
+        x[3] = csub(t1, t3);
+    } else if constexpr (R > 4) {
+        constexpr int Q = R / 2;
+        float2 e[Q], o[Q];
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            e[i] = x[2 * i];
+            o[i] = x[2 * i + 1];
+        }
+        dft<Q>(e);
+        dft<Q>(o);
+#pragma unroll
+        for (int k = 0; k < Q; ++k) {
+            const float2 t = twc<R>(o[k], k);
+            x[k] = cadd(e[k], t);
+            x[k + Q] = csub(e[k], t);
+        }
+    }
+}
+
+// LDS index with one padding slot per 16 complex values (breaks power-of-two strides).
+__host__ __device__ constexpr int padi(int e) { return e + (e >> 4); }
+// Stride between sequences stored back to back: padded length plus one odd slot, so that lanes
+// walking across sequences at a fixed position hit distinct banks.
+__host__ __device__ constexpr int seq_stride(int len) { return padi(len) + 1; }
+
+// Table of exp(-2 pi i x / 16384), x in [0, 16384), FP64-generated, in global memory.
+constexpr int TW14_LOG = 14;
+constexpr int TW14 = 1 << TW14_LOG;
+
+// Hides a pointer's value from the optimiser.  Twiddle tables are read-only, so their loads
+// would otherwise be merged across FFTs and hoisted across barriers into earlier stages, keeping
+// whole stages of twiddles live in registers; laundering the table pointer at the start of each
+// stage pins its loads after the preceding barrier.
+template <typename T>
+__device__ __forceinline__ const T* opaque_ptr(const T* p) {
+    uint64_t v = reinterpret_cast<uint64_t>(p);
+    asm volatile("" : "+s"(v));
+    return reinterpret_cast<const T*>(v);
+}
+
+// One in-place Stockham radix-R stage over NSEQ sequences of length 2^LG in LDS.
+// Reads all butterflies' inputs, barrier, writes, barrier.
+template <int R, int NS, int LG, int NSEQ, int SEQS, int NT>
+__device__ __forceinline__ void stockham_stage(float2* buf, const float2* __restrict__ tw14_arg) {
+    const float2* __restrict__ tw14 = opaque_ptr(tw14_arg);
+    constexpr int L = 1 << LG;
+    constexpr int BPS = L / R;
+    constexpr int NB = NSEQ * BPS;
+    constexpr int PER = (NB + NT - 1) / NT;
+    float2 v[PER][R];
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int b = tid + i * NT;
+        if ((NB % NT) == 0 || b < NB) {
+            const int seq = b / BPS, j = b % BPS;
+            const float2* base = buf + seq * SEQS;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[i][r] = base[padi(j + r * BPS)];
+            if constexpr (NS > 1) {
+                const int jj = j % NS;
+                constexpr int STEP = TW14 / (NS * R);
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[i][r] = cmul(v[i][r], tw14[(r * jj) * STEP]);
+            }
+            dft<R>(v[i]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int b = tid + i * NT;
+        if ((NB % NT) == 0 || b < NB) {
+            const int seq = b / BPS, j = b % BPS;
+            float2* base = buf + seq * SEQS;
+            const int d = (j / NS) * NS * R + (j % NS);
+#pragma unroll
+            for (int r = 0; r < R; ++r) base[padi(d + r * NS)] = v[i][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <int NS, int LG, int NSEQ, int SEQS, int NT>
+__device__ __forceinline__ void stockham_rest(float2* buf, const float2* __restrict__ tw14) {
+    if constexpr (NS < (1 << LG)) {
+        stockham_stage<16, NS, LG, NSEQ, SEQS, NT>(buf, tw14);
+        stockham_rest<NS * 16, LG, NSEQ, SEQS, NT>(buf, tw14);
+    }
+}
+
+// Forward FFT of NSEQ sequences of length 2^LG (LG <= 14) held in LDS at buf[s*SEQS + padi(e)].
+// Natural order in and out.  The caller must have synchronised after filling buf; the routine
+// ends with a barrier.
+template <int LG, int NSEQ, int SEQS, int NT>
+__device__ __forceinline__ void lds_fft(float2* buf, const float2* __restrict__ tw14) {
+    static_assert(LG >= 1 && LG <= TW14_LOG, "FFT length out of range");
+    constexpr int R0 = 1 << (LG % 4);
+    if constexpr (R0 > 1) {
+        stockham_stage<R0, 1, LG, NSEQ, SEQS, NT>(buf, tw14);
+        stockham_rest<R0, LG, NSEQ, SEQS, NT>(buf, tw14);
+    } else {
+        stockham_rest<1, LG, NSEQ, SEQS, NT>(buf, tw14);
+    }
+}
+
+// exp(-2 pi i x / L) from a two-level table: lo[x mod 2^14] * hi[x >> 14].
+__device__ __forceinline__ float2 tw_two_level(const float2* __restrict__ lo,
+                                               const float2* __restrict__ hi, uint32_t x) {
+    return cmul(lo[x & (TW14 - 1)], hi[x >> TW14_LOG]);
+}
+
+// Real-FFT split: given Z = FFT_N(x_even + i x_odd) at k and Z at (N-k) mod N, and w = exp(-2 pi i k/(2N)),
+// returns X[k] of the length-2N real transform.
+__device__ __forceinline__ float2 r2c_split(float2 zk, float2 zp, float2 w) {
+    const float2 zc = conjf2(zp);
+    const float2 e = cscale(cadd(zk, zc), 0.5f);
+    const float2 d = cscale(csub(zk, zc), 0.5f);
+    const float2 t = cmul(w, d);
+    return make_float2(e.x + t.y, e.y - t.x);  // e - i*t
+}
+// Inverse split: from P[k], P[N-k] of a Hermitian length-2N spectrum, returns
+// Q[k] = FFT_N(c_even + i c_odd)[k].
+__device__ __forceinline__ float2 c2r_split(float2 pk, float2 pp, float2 w) {
+    const float2 pc = conjf2(pp);
+    const float2 e = cscale(cadd(pk, pc), 0.5f);
+    const float2 d = cscale(csub(pk, pc), 0.5f);
+    const float2 t = cmul(conjf2(w), d);
+    return make_float2(e.x - t.y, e.y + t.x);  // e + i*t
+}
+
+// Wave-level max of a non-negative float published to a device float with an atomic max
+// on its bit pattern (non-negative IEEE floats order like their unsigned bit patterns).
+__device__ __forceinline__ void margin_publish(float v, float* margin) {
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(reinterpret_cast<unsigned int*>(margin), __float_as_uint(v));
+}
+
+// Blockid remap so that consecutive logical ids land on the same XCD (workgroups are dealt to
+// the 8 XCDs round robin): logical = (b % 8) * (G / 8) + b / 8 when G is a multiple of 8.
+__device__ __forceinline__ uint32_t xcd_logical(uint32_t b, uint32_t G) {
+    if ((G & 7u) != 0u) return b;
+    return (b & 7u) * (G >> 3) + (b >> 3);
+}
+
+}  // namespace pa

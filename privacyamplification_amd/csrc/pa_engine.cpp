@@ -1,0 +1,651 @@
+// pa_engine.cpp -- host side of the MI355X Toeplitz-hashing engine: the C ABI of
+// include/pa_engine.h, block planning, device workspaces, twiddle tables and resident seeds.
+//
+// The reference runs one block per mainloop iteration with a synchronise after every step
+// (PrivacyAmplification.cu:1686-2118) and re-plans its FFTs on every mainloop entry
+// (:1712-1756).  Here a block is a short stream-ordered kernel sequence on the engine's stream;
+// workspaces and twiddle tables are created once per transform length and kept.
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "../../include/pa_engine.h"
+#include "pa_geom.hpp"
+#include "pa_kernels.hpp"
+
+namespace {
+
+using pa::Geom;
+
+#define PA_HIP(expr)                                              \
+    do {                                                          \
+        hipError_t e_ = (expr);                                   \
+        if (e_ != hipSuccess) {                                   \
+            set_error(e_, #expr, __FILE__, __LINE__);             \
+            return PA_ERR_DEVICE;                                 \
+        }                                                         \
+    } while (0)
+
+thread_local char g_last_error[512] = "";
+void set_error(hipError_t e, const char* what, const char* file, int line) {
+    snprintf(g_last_error, sizeof(g_last_error), "%s: %s (%s:%d)", what, hipGetErrorString(e), file, line);
+}
+
+bool is_pow2(uint32_t x) { return x && !(x & (x - 1)); }
+uint32_t log2u(uint64_t x) {
+    uint32_t r = 0;
+    while ((1ull << r) < x) ++r;
+    return r;
+}
+
+// Plans the transform for a block: see pa_geom.hpp.  Returns PA_OK or an error status.
+int make_geom(uint32_t n, uint32_t m, uint32_t flags, uint32_t reverse_endian, uint32_t max_n_log2,
+              Geom* g) {
+    if (flags & ~3u) return PA_ERR_ARG;
+    if (n < 64 || (n % 32) || m == 0 || (m % 32) || m >= n) return PA_ERR_SIZE;
+    const bool pow2 = is_pow2(n);
+    const bool compress = (flags & PA_COMPRESS) != 0;
+    if (!pow2 && !compress) return PA_ERR_UNSUPPORTED;  // n/2-bit output is defined for 2^k only
+    const uint32_t L = pow2 ? n : (1u << log2u(n));
+    if (log2u(L) > pa::MAX_L_LOG || log2u(L) > max_n_log2) return PA_ERR_SIZE;
+    Geom x{};
+    x.n = n;
+    x.m = m;
+    x.h = n - m;
+    x.kbits = x.h + 1;
+    x.L = L;
+    x.N = L / 2;
+    x.lgN = log2u(x.N);
+    if (x.lgN <= (uint32_t)pa::SMALL_MAX_LOG_N) {
+        x.N1 = 1;
+        x.N2 = x.N;
+        x.lg1 = 0;
+        x.lg2 = x.lgN;
+        x.M = 1;
+        x.D = 1;
+        x.W = 1;
+    } else {
+        x.lg2 = pa::ROW_LOG;
+        x.N2 = 1u << x.lg2;
+        x.lg1 = x.lgN - x.lg2;
+        x.N1 = 1u << x.lg1;
+        x.M = x.N1 < 512 ? x.N1 : 512;
+        x.D = x.N1 / x.M;
+        x.W = (uint32_t)pa::inv_cols_width((int)x.lg1);
+    }
+    x.out_bits = compress ? m : n / 2;
+    x.out_off = pow2 ? 0 : x.h;
+    x.seed_words = n / 32;
+    x.seed_rot = pow2 ? 0 : m / 32;
+    x.key_hw = x.h / 32;
+    x.do_xor = (flags & PA_XOR_KEY_REST) ? 1u : 0u;
+    x.reverse_endian = reverse_endian ? 1u : 0u;
+    x.inv_N = 1.0f / (float)x.N;
+    *g = x;
+    return PA_OK;
+}
+
+uint32_t key_words_for(const Geom& g) { return g.do_xor ? g.n / 32 + 1 : g.key_hw + 1; }
+
+// Per transform length: twiddles and the device scratch of one block.
+struct Workspace {
+    uint32_t L = 0;
+    float2* lo = nullptr;      // 2^14 entries
+    float2* hi = nullptr;      // max(1, L / 2^14) entries
+    float2* Yk = nullptr;      // N complex (key rows / R')
+    float2* Ys = nullptr;      // N+1 complex (seed rows / split seed spectrum)
+    uint32_t* tbk = nullptr;   // N/D words
+    uint32_t* tbs = nullptr;
+    uint8_t* nib = nullptr;    // L/4 bytes
+    uint32_t* counts = nullptr;  // [0] key popcount, [1] seed popcount
+    uint32_t* d_key = nullptr;   // staging for the host-buffer API
+    uint32_t* d_seed = nullptr;
+    uint32_t* d_out = nullptr;
+    uint64_t bytes = 0;
+};
+
+struct Seed {
+    Geom g{};
+    float2* spec = nullptr;    // N+1 complex split spectrum
+    uint32_t* ws = nullptr;    // device popcount
+    int32_t reuse = 0;
+    uint64_t bytes = 0;
+};
+
+struct Ticket {
+    hipEvent_t done = nullptr;
+    uint8_t* host_out = nullptr;
+    uint32_t* dev_out = nullptr;  // owned output staging
+    uint32_t* dev_key = nullptr;
+    uint32_t* dev_seed = nullptr;
+    size_t out_bytes = 0;
+};
+
+}  // namespace
+
+struct pa_engine {
+    pa_config cfg{};
+    hipStream_t stream = nullptr;
+    float2* tw14 = nullptr;
+    std::map<uint32_t, std::unique_ptr<Workspace>> ws;
+    std::map<pa_seed_id, Seed> seeds;
+    std::map<pa_ticket, Ticket> tickets;
+    pa_seed_id next_seed = 1;
+    pa_ticket next_ticket = 1;
+    uint64_t used_bytes = 0;
+    std::mutex mu;
+    // stage timing
+    bool timing = false;
+    struct Pending { int stage; hipEvent_t a, b; };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> event_pool;
+    double stage_ms[16] = {};
+    uint32_t stage_count[16] = {};
+};
+
+namespace {
+
+int dev_alloc(pa_engine* e, void** p, size_t bytes) {
+    if (e->cfg.hbm_budget_bytes && e->used_bytes + bytes > e->cfg.hbm_budget_bytes) return PA_ERR_NOMEM;
+    hipError_t r = hipMalloc(p, bytes);
+    if (r != hipSuccess) {
+        set_error(r, "hipMalloc", __FILE__, __LINE__);
+        (void)hipGetLastError();
+        return PA_ERR_NOMEM;
+    }
+    e->used_bytes += bytes;
+    return PA_OK;
+}
+void dev_free(pa_engine* e, void* p, size_t bytes) {
+    if (!p) return;
+    (void)hipFree(p);
+    e->used_bytes -= bytes;
+}
+
+std::vector<float2> twiddles(uint64_t L, uint64_t count, uint64_t stride) {
+    std::vector<float2> t(count);
+    for (uint64_t i = 0; i < count; ++i) {
+        const uint64_t x = (i * stride) % L;
+        const double a = -2.0 * M_PI * (double)x / (double)L;
+        t[i] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    return t;
+}
+
+uint64_t workspace_bytes(const Geom& g) {
+    const uint64_t N = g.N;
+    uint64_t b = (16384 + (N > 8192 ? g.L / 16384 : 1)) * sizeof(float2);
+    if (g.N1 > 1) {
+        b += N * sizeof(float2) + (N + 1) * sizeof(float2);
+        b += 2 * (N / g.D) * 4 + g.L / 4 + 256;
+    }
+    b += 256 + (g.n / 32 + 2) * 4 * 2 + (g.n / 16 + 4);
+    return b;
+}
+
+int get_workspace(pa_engine* e, const Geom& g, Workspace** out) {
+    auto it = e->ws.find(g.L);
+    if (it != e->ws.end()) {
+        *out = it->second.get();
+        return PA_OK;
+    }
+    auto w = std::make_unique<Workspace>();
+    w->L = g.L;
+    int rc;
+    const uint64_t hi_n = g.L > 16384 ? g.L / 16384 : 1;
+    const size_t key_bytes = (size_t)(g.L / 32 + 2) * 4, out_bytes = (size_t)(g.L / 16 + 16);
+    struct A { void** p; size_t b; };
+    std::vector<A> allocs = {
+        {(void**)&w->lo, 16384 * sizeof(float2)},
+        {(void**)&w->hi, hi_n * sizeof(float2)},
+        {(void**)&w->counts, 256},
+        {(void**)&w->d_key, key_bytes},
+        {(void**)&w->d_seed, key_bytes},
+        {(void**)&w->d_out, out_bytes},
+    };
+    if (g.N1 > 1) {
+        allocs.push_back({(void**)&w->Yk, (size_t)g.N * sizeof(float2)});
+        allocs.push_back({(void**)&w->Ys, (size_t)(g.N + 1) * sizeof(float2)});
+        allocs.push_back({(void**)&w->tbk, (size_t)(g.N / g.D) * 4});
+        allocs.push_back({(void**)&w->tbs, (size_t)(g.N / g.D) * 4});
+        allocs.push_back({(void**)&w->nib, (size_t)g.L / 4 + 64});
+    }
+    for (auto& a : allocs) {
+        rc = dev_alloc(e, a.p, a.b);
+        if (rc != PA_OK) {
+            for (auto& b : allocs)
+                if (*b.p) dev_free(e, *b.p, b.b);
+            return rc;
+        }
+        w->bytes += a.b;
+    }
+    auto lo = twiddles(g.L, 16384, 1);
+    auto hi = twiddles(g.L, hi_n, 16384);
+    PA_HIP(hipMemcpy(w->lo, lo.data(), lo.size() * sizeof(float2), hipMemcpyHostToDevice));
+    PA_HIP(hipMemcpy(w->hi, hi.data(), hi.size() * sizeof(float2), hipMemcpyHostToDevice));
+    *out = w.get();
+    e->ws.emplace(g.L, std::move(w));
+    return PA_OK;
+}
+
+pa::Tables tables(pa_engine* e, const Workspace* w) { return pa::Tables{e->tw14, w->lo, w->hi}; }
+
+hipStream_t stream_of(pa_engine* e, void* s) { return s ? (hipStream_t)s : e->stream; }
+
+enum Stage { ST_SMALL, ST_ZERO, ST_TRANSPOSE_KEY, ST_FWD_COLS_KEY, ST_TRANSPOSE_SEED, ST_FWD_COLS_SEED,
+             ST_ROWS_SEEDPREP, ST_ROWS, ST_INV_COLS, ST_PACK, ST_COUNT };
+const char* const kStageNames[ST_COUNT] = {
+    "k_small", "k_zero_u32", "k_transpose_bits(key)", "k_fwd_cols(key)", "k_transpose_bits(seed)",
+    "k_fwd_cols(seed)", "k_rows(seedprep)", "k_rows", "k_inv_cols", "k_pack"};
+
+hipEvent_t pool_event(pa_engine* e) {
+    if (!e->event_pool.empty()) {
+        hipEvent_t ev = e->event_pool.back();
+        e->event_pool.pop_back();
+        return ev;
+    }
+    hipEvent_t ev = nullptr;
+    if (hipEventCreate(&ev) != hipSuccess) return nullptr;
+    return ev;
+}
+
+// Runs one launch, bracketed by events when stage timing is on.
+template <typename F>
+hipError_t timed(pa_engine* e, int stage, hipStream_t s, F&& launch) {
+    if (!e->timing) return launch();
+    hipEvent_t a = pool_event(e), b = pool_event(e);
+    if (!a || !b) return hipErrorOutOfMemory;
+    hipError_t r = hipEventRecord(a, s);
+    if (r != hipSuccess) return r;
+    r = launch();
+    if (r != hipSuccess) return r;
+    r = hipEventRecord(b, s);
+    e->pending.push_back({stage, a, b});
+    return r;
+}
+
+// The hot path on device buffers.  sspec/ws_dev != null selects the resident seed.
+int run_device(pa_engine* e, const Geom& g, Workspace* w, const uint32_t* key, const uint32_t* seed,
+               const float2* sspec, const uint32_t* ws_dev, uint32_t* out, hipStream_t s, float* margin) {
+    const pa::Tables t = tables(e, w);
+    if (g.N1 == 1) {
+        PA_HIP(timed(e, ST_SMALL, s, [&] {
+            return pa::launch_small(key, seed, sspec, ws_dev, out, g, t, margin, nullptr, nullptr, s);
+        }));
+        return PA_OK;
+    }
+    PA_HIP(timed(e, ST_ZERO, s, [&] { return pa::launch_zero_u32(w->counts, 2, s); }));
+    PA_HIP(timed(e, ST_TRANSPOSE_KEY, s, [&] { return pa::launch_transpose(key, w->tbk, w->counts + 0, g, 0, s); }));
+    PA_HIP(timed(e, ST_FWD_COLS_KEY, s, [&] { return pa::launch_fwd_cols(w->tbk, w->Yk, g, t, s); }));
+    if (!sspec) {
+        PA_HIP(timed(e, ST_TRANSPOSE_SEED, s, [&] { return pa::launch_transpose(seed, w->tbs, w->counts + 1, g, 1, s); }));
+        PA_HIP(timed(e, ST_FWD_COLS_SEED, s, [&] { return pa::launch_fwd_cols(w->tbs, w->Ys, g, t, s); }));
+        PA_HIP(timed(e, ST_ROWS_SEEDPREP, s, [&] { return pa::launch_rows(pa::ROWS_SEEDPREP, w->Yk, w->Ys, g, t, s); }));
+        sspec = w->Ys;
+        ws_dev = w->counts + 1;
+    }
+    float2* S = const_cast<float2*>(sspec);
+    PA_HIP(timed(e, ST_ROWS, s, [&] { return pa::launch_rows(pa::ROWS_STATIC, w->Yk, S, g, t, s); }));
+    PA_HIP(timed(e, ST_INV_COLS, s, [&] { return pa::launch_inv_cols(w->Yk, w->nib, g, w->counts + 0, ws_dev, t, margin, s); }));
+    PA_HIP(timed(e, ST_PACK, s, [&] { return pa::launch_pack(w->nib, key, out, g, s); }));
+    return PA_OK;
+}
+
+int prepare(pa_engine* e, uint32_t n, uint32_t m, uint32_t flags, Geom* g, Workspace** w) {
+    int rc = make_geom(n, m, flags, e->cfg.out_reverse_endian, e->cfg.max_n_log2, g);
+    if (rc != PA_OK) return rc;
+    return get_workspace(e, *g, w);
+}
+
+size_t out_bytes_of(const Geom& g) { return g.out_bits / 8; }
+
+}  // namespace
+
+extern "C" {
+
+int pa_config_default(pa_config* cfg) {
+    if (!cfg) return PA_ERR_ARG;
+    memset(cfg, 0, sizeof(*cfg));
+    cfg->device = 0;
+    cfg->max_n_log2 = 27;
+    cfg->reduction_exp = 11;
+    cfg->pre_mul_reduction_exp = 5;
+    cfg->hbm_budget_bytes = 64ull << 30;
+    cfg->out_reverse_endian = 1;
+    return PA_OK;
+}
+
+int pa_create(const pa_config* cfg, pa_engine** out) {
+    if (!out) return PA_ERR_ARG;
+    auto e = std::make_unique<pa_engine>();
+    if (cfg) e->cfg = *cfg; else pa_config_default(&e->cfg);
+    if (e->cfg.max_n_log2 == 0) e->cfg.max_n_log2 = 27;
+    PA_HIP(hipSetDevice(e->cfg.device));
+    PA_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    int rc = dev_alloc(e.get(), (void**)&e->tw14, 16384 * sizeof(float2));
+    if (rc != PA_OK) return rc;
+    auto tw = twiddles(16384, 16384, 1);
+    PA_HIP(hipMemcpy(e->tw14, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice));
+    *out = e.release();
+    return PA_OK;
+}
+
+int pa_destroy(pa_engine* e) {
+    if (!e) return PA_ERR_ARG;
+    (void)hipSetDevice(e->cfg.device);
+    (void)hipStreamSynchronize(e->stream);
+    for (auto& kv : e->tickets) {
+        Ticket& t = kv.second;
+        if (t.done) (void)hipEventDestroy(t.done);
+        (void)hipFree(t.dev_out);
+        (void)hipFree(t.dev_key);
+        (void)hipFree(t.dev_seed);
+    }
+    for (auto& kv : e->seeds) {
+        (void)hipFree(kv.second.spec);
+        (void)hipFree(kv.second.ws);
+    }
+    for (auto& kv : e->ws) {
+        Workspace* w = kv.second.get();
+        void* ps[] = {w->lo, w->hi, w->Yk, w->Ys, w->tbk, w->tbs, w->nib, w->counts, w->d_key, w->d_seed, w->d_out};
+        for (void* p : ps) (void)hipFree(p);
+    }
+    for (auto& p : e->pending) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    for (hipEvent_t ev : e->event_pool) (void)hipEventDestroy(ev);
+    (void)hipFree(e->tw14);
+    (void)hipStreamDestroy(e->stream);
+    delete e;
+    return PA_OK;
+}
+
+uint64_t pa_workspace_bytes(uint32_t n, uint32_t m, uint32_t flags) {
+    Geom g;
+    if (make_geom(n, m, flags, 1, 31, &g) != PA_OK) return 0;
+    return workspace_bytes(g);
+}
+
+int pa_seed_put(pa_engine* e, const uint32_t* seed, uint32_t n, uint32_t m, int32_t reuse, pa_seed_id* id) {
+    if (!e || !seed || !id) return PA_ERR_ARG;
+    std::lock_guard<std::mutex> lk(e->mu);
+    Geom g;
+    Workspace* w;
+    int rc = prepare(e, n, m, PA_DEFAULT_FLAGS, &g, &w);
+    if (rc != PA_OK) return rc;
+    Seed sd;
+    sd.g = g;
+    sd.reuse = reuse;
+    sd.bytes = (uint64_t)(g.N + 1) * sizeof(float2) + 256;
+    rc = dev_alloc(e, (void**)&sd.spec, (size_t)(g.N + 1) * sizeof(float2));
+    if (rc != PA_OK) return rc;
+    rc = dev_alloc(e, (void**)&sd.ws, 256);
+    if (rc != PA_OK) {
+        dev_free(e, sd.spec, (size_t)(g.N + 1) * sizeof(float2));
+        return rc;
+    }
+    hipStream_t s = e->stream;
+    PA_HIP(hipMemcpyAsync(w->d_seed, seed, (size_t)(n / 32) * 4, hipMemcpyHostToDevice, s));
+    const pa::Tables t = tables(e, w);
+    if (g.N1 == 1) {
+        PA_HIP(pa::launch_small(nullptr, w->d_seed, nullptr, nullptr, nullptr, g, t, nullptr, sd.spec, sd.ws, s));
+    } else {
+        PA_HIP(pa::launch_zero_u32(sd.ws, 1, s));
+        PA_HIP(pa::launch_transpose(w->d_seed, w->tbs, sd.ws, g, 1, s));
+        PA_HIP(pa::launch_fwd_cols(w->tbs, sd.spec, g, t, s));
+        PA_HIP(pa::launch_rows(pa::ROWS_SEEDPREP, nullptr, sd.spec, g, t, s));
+    }
+    PA_HIP(hipStreamSynchronize(s));
+    *id = e->next_seed++;
+    e->seeds[*id] = sd;
+    return PA_OK;
+}
+
+int pa_seed_drop(pa_engine* e, pa_seed_id id) {
+    if (!e) return PA_ERR_ARG;
+    std::lock_guard<std::mutex> lk(e->mu);
+    auto it = e->seeds.find(id);
+    if (it == e->seeds.end()) return PA_ERR_NOT_FOUND;
+    (void)hipStreamSynchronize(e->stream);
+    dev_free(e, it->second.spec, (size_t)(it->second.g.N + 1) * sizeof(float2));
+    dev_free(e, it->second.ws, 256);
+    e->seeds.erase(it);
+    return PA_OK;
+}
+
+static int seed_lookup(pa_engine* e, pa_seed_id id, const Geom& g, const Seed** out) {
+    auto it = e->seeds.find(id);
+    if (it == e->seeds.end()) return PA_ERR_NOT_FOUND;
+    const Geom& sg = it->second.g;
+    if (sg.n != g.n || sg.L != g.L || sg.seed_rot != g.seed_rot) return PA_ERR_ARG;
+    *out = &it->second;
+    return PA_OK;
+}
+
+int pa_block_device(pa_engine* e, const uint32_t* d_key, const uint32_t* d_seed, uint32_t n, uint32_t m,
+                    uint32_t flags, uint8_t* d_out, void* stream, float* d_margin) {
+    if (!e || !d_key || !d_seed || !d_out) return PA_ERR_ARG;
+    std::lock_guard<std::mutex> lk(e->mu);
+    Geom g;
+    Workspace* w;
+    int rc = prepare(e, n, m, flags, &g, &w);
+    if (rc != PA_OK) return rc;
+    return run_device(e, g, w, d_key, d_seed, nullptr, nullptr, (uint32_t*)d_out, stream_of(e, stream), d_margin);
+}
+
+int pa_block_device_seeded(pa_engine* e, const uint32_t* d_key, pa_seed_id id, uint32_t n, uint32_t m,
+                           uint32_t flags, uint8_t* d_out, void* stream, float* d_margin) {
+    if (!e || !d_key || !d_out) return PA_ERR_ARG;
+    std::lock_guard<std::mutex> lk(e->mu);
+    Geom g;
+    Workspace* w;
+    int rc = prepare(e, n, m, flags, &g, &w);
+    if (rc != PA_OK) return rc;
+    const Seed* sd;
+    rc = seed_lookup(e, id, g, &sd);
+    if (rc != PA_OK) return rc;
+    return run_device(e, g, w, d_key, nullptr, sd->spec, sd->ws, (uint32_t*)d_out, stream_of(e, stream), d_margin);
+}
+
+static int run_host(pa_engine* e, const uint32_t* key, const uint32_t* seed, pa_seed_id id, uint32_t n,
+                    uint32_t m, uint32_t flags, uint8_t* out) {
+    if (!e || !key || !out || (!seed && !id)) return PA_ERR_ARG;
+    std::lock_guard<std::mutex> lk(e->mu);
+    Geom g;
+    Workspace* w;
+    int rc = prepare(e, n, m, flags, &g, &w);
+    if (rc != PA_OK) return rc;
+    const Seed* sd = nullptr;
+    if (!seed) {
+        rc = seed_lookup(e, id, g, &sd);
+        if (rc != PA_OK) return rc;
+    }
+    hipStream_t s = e->stream;
+    const uint32_t kw = key_words_for(g);
+    // zero the word after the key so key-rest reads past a NoXOR key stay defined
+    PA_HIP(hipMemsetAsync(w->d_key, 0, (size_t)(g.n / 32 + 2) * 4, s));
+    PA_HIP(hipMemcpyAsync(w->d_key, key, (size_t)kw * 4, hipMemcpyHostToDevice, s));
+    if (seed) PA_HIP(hipMemcpyAsync(w->d_seed, seed, (size_t)(n / 32) * 4, hipMemcpyHostToDevice, s));
+    rc = run_device(e, g, w, w->d_key, seed ? w->d_seed : nullptr, sd ? sd->spec : nullptr,
+                    sd ? sd->ws : nullptr, w->d_out, s, nullptr);
+    if (rc != PA_OK) return rc;
+    PA_HIP(hipMemcpyAsync(out, w->d_out, out_bytes_of(g), hipMemcpyDeviceToHost, s));
+    PA_HIP(hipStreamSynchronize(s));
+    return PA_OK;
+}
+
+int pa_run_block(pa_engine* e, const uint32_t* key, const uint32_t* seed, uint32_t n, uint32_t m,
+                 uint32_t flags, uint8_t* out) {
+    if (!seed) return PA_ERR_ARG;
+    return run_host(e, key, seed, 0, n, m, flags, out);
+}
+
+int pa_run_block_seeded(pa_engine* e, const uint32_t* key, pa_seed_id id, uint32_t n, uint32_t m,
+                        uint32_t flags, uint8_t* out) {
+    if (!id) return PA_ERR_ARG;
+    return run_host(e, key, nullptr, id, n, m, flags, out);
+}
+
+int pa_submit(pa_engine* e, const uint32_t* key, const uint32_t* seed, pa_seed_id id, uint32_t n, uint32_t m,
+              uint32_t flags, uint8_t* out, pa_ticket* ticket) {
+    if (!e || !key || !out || !ticket || (!seed && !id)) return PA_ERR_ARG;
+    std::lock_guard<std::mutex> lk(e->mu);
+    Geom g;
+    Workspace* w;
+    int rc = prepare(e, n, m, flags, &g, &w);
+    if (rc != PA_OK) return rc;
+    const Seed* sd = nullptr;
+    if (!seed) {
+        rc = seed_lookup(e, id, g, &sd);
+        if (rc != PA_OK) return rc;
+    }
+    hipStream_t s = e->stream;
+    Ticket t;
+    t.host_out = out;
+    t.out_bytes = out_bytes_of(g);
+    const size_t kbytes = (size_t)(g.n / 32 + 2) * 4;
+    PA_HIP(hipMalloc(&t.dev_key, kbytes));
+    PA_HIP(hipMalloc(&t.dev_out, t.out_bytes + 16));
+    if (seed) PA_HIP(hipMalloc(&t.dev_seed, (size_t)(n / 32) * 4));
+    PA_HIP(hipMemsetAsync(t.dev_key, 0, kbytes, s));
+    PA_HIP(hipMemcpyAsync(t.dev_key, key, (size_t)key_words_for(g) * 4, hipMemcpyHostToDevice, s));
+    if (seed) PA_HIP(hipMemcpyAsync(t.dev_seed, seed, (size_t)(n / 32) * 4, hipMemcpyHostToDevice, s));
+    rc = run_device(e, g, w, t.dev_key, t.dev_seed, sd ? sd->spec : nullptr, sd ? sd->ws : nullptr,
+                    t.dev_out, s, nullptr);
+    if (rc != PA_OK) return rc;
+    PA_HIP(hipMemcpyAsync(out, t.dev_out, t.out_bytes, hipMemcpyDeviceToHost, s));
+    PA_HIP(hipEventCreateWithFlags(&t.done, hipEventDisableTiming));
+    PA_HIP(hipEventRecord(t.done, s));
+    *ticket = e->next_ticket++;
+    e->tickets[*ticket] = t;
+    return PA_OK;
+}
+
+int pa_wait(pa_engine* e, pa_ticket ticket, int64_t timeout_ns) {
+    if (!e) return PA_ERR_ARG;
+    hipEvent_t ev;
+    {
+        std::lock_guard<std::mutex> lk(e->mu);
+        auto it = e->tickets.find(ticket);
+        if (it == e->tickets.end()) return PA_ERR_NOT_FOUND;
+        ev = it->second.done;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    while (true) {
+        hipError_t q = hipEventQuery(ev);
+        if (q == hipSuccess) break;
+        if (q != hipErrorNotReady) {
+            set_error(q, "hipEventQuery", __FILE__, __LINE__);
+            return PA_ERR_DEVICE;
+        }
+        if (timeout_ns >= 0) {
+            const auto dt = std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - t0);
+            if (dt.count() > timeout_ns) return PA_ERR_TIMEOUT;
+        }
+        std::this_thread::yield();
+    }
+    std::lock_guard<std::mutex> lk(e->mu);
+    auto it = e->tickets.find(ticket);
+    Ticket t = it->second;
+    e->tickets.erase(it);
+    (void)hipEventDestroy(t.done);
+    (void)hipFree(t.dev_out);
+    (void)hipFree(t.dev_key);
+    (void)hipFree(t.dev_seed);
+    return PA_OK;
+}
+
+int pa_stage_timing(pa_engine* e, int enable) {
+    if (!e) return PA_ERR_ARG;
+    std::lock_guard<std::mutex> lk(e->mu);
+    e->timing = enable != 0;
+    return PA_OK;
+}
+
+int pa_stage_times(pa_engine* e, const char** names, double* ms, uint32_t* counts, uint32_t* n_stages) {
+    if (!e || !n_stages) return PA_ERR_ARG;
+    std::lock_guard<std::mutex> lk(e->mu);
+    for (auto& p : e->pending) {
+        PA_HIP(hipEventSynchronize(p.b));
+        float dt = 0.f;
+        PA_HIP(hipEventElapsedTime(&dt, p.a, p.b));
+        e->stage_ms[p.stage] += dt;
+        e->stage_count[p.stage] += 1;
+        e->event_pool.push_back(p.a);
+        e->event_pool.push_back(p.b);
+    }
+    e->pending.clear();
+    uint32_t cap = *n_stages, k = 0;
+    for (int st = 0; st < ST_COUNT; ++st) {
+        if (e->stage_count[st] == 0) continue;
+        if (k < cap) {
+            if (names) names[k] = kStageNames[st];
+            if (ms) ms[k] = e->stage_ms[st];
+            if (counts) counts[k] = e->stage_count[st];
+        }
+        ++k;
+        e->stage_ms[st] = 0.0;
+        e->stage_count[st] = 0;
+    }
+    *n_stages = k < cap ? k : cap;
+    return PA_OK;
+}
+
+int pa_synth_bits(uint64_t seed, double p, uint64_t nbits, uint32_t* out) {
+    if (!out || p < 0.0 || p > 1.0) return PA_ERR_ARG;
+    auto rotl = [](uint64_t x, int k) { return (x << k) | (x >> (64 - k)); };
+    uint64_t sm = seed, s[4];
+    for (int i = 0; i < 4; ++i) {
+        uint64_t z = (sm += 0x9e3779b97f4a7c15ull);
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        s[i] = z ^ (z >> 31);
+    }
+    auto next = [&]() {
+        const uint64_t r = rotl(s[1] * 5, 7) * 9, t = s[1] << 17;
+        s[2] ^= s[0]; s[3] ^= s[1]; s[1] ^= s[2]; s[0] ^= s[3]; s[2] ^= t; s[3] = rotl(s[3], 45);
+        return r;
+    };
+    const uint64_t nw = (nbits + 31) / 32;
+    if (p == 0.5) {
+        for (uint64_t i = 0; i < nw; i += 2) {
+            const uint64_t r = next();
+            out[i] = (uint32_t)(r >> 32);
+            if (i + 1 < nw) out[i + 1] = (uint32_t)r;
+        }
+    } else {
+        memset(out, 0, nw * 4);
+        for (uint64_t b = 0; b < nbits; ++b)
+            if ((double)(next() >> 11) * 0x1.0p-53 < p) out[b >> 5] |= 1u << (31 - (b & 31));
+    }
+    if (nbits & 31) out[nw - 1] &= ~((1u << (32 - (nbits & 31))) - 1u);
+    return PA_OK;
+}
+
+const char* pa_strerror(int status) {
+    switch (status) {
+    case PA_OK: return "ok";
+    case PA_ERR_ARG: return "invalid argument";
+    case PA_ERR_SIZE: return "block size outside the supported family";
+    case PA_ERR_DEVICE: return g_last_error[0] ? g_last_error : "HIP runtime error";
+    case PA_ERR_NOMEM: return "device memory or HBM budget exhausted";
+    case PA_ERR_TIMEOUT: return "timed out";
+    case PA_ERR_NOT_FOUND: return "unknown seed or ticket";
+    case PA_ERR_UNSUPPORTED: return "unsupported request";
+    default: return "unknown status";
+    }
+}
+
+const char* pa_version(void) { return "pa-mi355x 0.1 (gfx950)"; }
+
+}  // extern "C"

@@ -1,0 +1,37 @@
+// pa_geom.hpp -- block geometry shared by the host planner and the kernels.
+//
+// A block (n, m, flags) is evaluated as a circular convolution of length L (a power of two, L >= n)
+// of the key x (kbits = h+1 bits, h = n-m) with the seed t (n bits, rotated left by m when L > n),
+// read at conv indices [out_off, out_off + out_bits).  For power-of-two n, L = n and out_off = 0,
+// which is the reference's own circular form y[p] = XOR_j x[j] s[(p-j) mod n].
+//
+// The length-L real transform is a length-N complex transform (N = L/2, z[t] = x[2t] + i x[2t+1])
+// evaluated as a four-step FFT with N = N1 * N2 (t = t2 + N2*t1, k = k1 + N1*k2):
+//   pass A  (fwd_cols)  column FFTs of length N1, decimated by D classes of k1 (M = N1/D per class),
+//                       16 columns per workgroup, input from the bit-transposed key/seed;
+//   pass B  (rows)      row FFTs of length N2 on rows k1 and N1-k1, real split, DC zeroing, complex
+//                       product, inverse split, inverse row FFTs;
+//   pass A' (inv_cols)  inverse column FFTs of length N1, W columns per workgroup, round-mod-2 epilogue.
+// N <= 8192 (L <= 2^14) runs fully on chip in one workgroup (small path).
+#pragma once
+#include <stdint.h>
+
+namespace pa {
+
+constexpr int ROW_LOG = 13;            // N2 = 8192 for the four-step path
+constexpr int SMALL_MAX_LOG_N = 13;    // N <= 8192 -> single-workgroup path
+constexpr int MAX_L_LOG = 27;          // largest transform served (L = 2^27)
+
+struct Geom {
+    uint32_t n, m, h, kbits;           // block shape; kbits = h + 1
+    uint32_t L, N, lgN;                // transform length, half length, log2(N)
+    uint32_t N1, N2, lg1, lg2;         // four-step split (N1 = 1 on the small path)
+    uint32_t M, D, W;                  // pass A class length / classes, pass A' columns per workgroup
+    uint32_t out_bits, out_off;        // output window in conv indices
+    uint32_t seed_words, seed_rot;     // n/32, rotation of the seed in words (m/32 when L > n)
+    uint32_t key_hw;                   // h/32: index of the key word masked to its MSB
+    uint32_t do_xor, reverse_endian;
+    float inv_N;
+};
+
+}  // namespace pa

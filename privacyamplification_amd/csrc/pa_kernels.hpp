@@ -1,0 +1,45 @@
+// pa_kernels.hpp -- host-side launchers of the gfx950 kernels (internal to libpa_hip.so).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pa_geom.hpp"
+
+namespace pa {
+
+enum { ROWS_STATIC = 1, ROWS_SEEDPREP = 2 };
+constexpr int ROWS_NT = 512;  // threads of the row pass (256-VGPR budget per lane)
+
+// Device twiddle tables: tw14[x] = exp(-2 pi i x / 2^14); lo/hi: exp(-2 pi i x / L) split as
+// lo[x mod 2^14] * hi[x >> 14].
+struct Tables {
+    const float2* tw14;
+    const float2* lo;
+    const float2* hi;
+};
+
+// Columns per workgroup in pass A' (tile of W x N1 complex <= 16384 values) and its threads.
+__host__ __device__ constexpr int inv_cols_width(int lg1) {
+    return (14 - lg1) >= 4 ? 16 : (1 << (14 - lg1));
+}
+__host__ __device__ constexpr int clamp_threads(int t) { return t < 64 ? 64 : (t > 1024 ? 1024 : t); }
+__host__ __device__ constexpr int inv_cols_threads(int lg1) {
+    return clamp_threads(inv_cols_width(lg1) * (1 << lg1) / 16);
+}
+__host__ __device__ constexpr int small_threads(int lgn) { return clamp_threads(2 * (1 << lgn) / 16); }
+
+hipError_t launch_zero_u32(uint32_t* p, uint32_t count, hipStream_t s);
+hipError_t launch_transpose(const uint32_t* bits, uint32_t* tb, uint32_t* count, const Geom& g,
+                            int is_seed, hipStream_t s);
+hipError_t launch_fwd_cols(const uint32_t* tb, float2* Y, const Geom& g, const Tables& t, hipStream_t s);
+// ROWS_SEEDPREP: S <- split spectrum of the seed rows held in S (in place, N+1 values).
+// ROWS_STATIC:   Yk <- R' from the key rows in Yk and the split seed spectrum S.
+hipError_t launch_rows(int mode, float2* Yk, float2* S, const Geom& g, const Tables& t, hipStream_t s);
+hipError_t launch_inv_cols(const float2* R, uint8_t* nib, const Geom& g, const uint32_t* wk,
+                           const uint32_t* ws, const Tables& t, float* margin, hipStream_t s);
+hipError_t launch_pack(const uint8_t* nib, const uint32_t* key, uint32_t* out, const Geom& g, hipStream_t s);
+hipError_t launch_small(const uint32_t* key, const uint32_t* seed, const float2* sspec,
+                        const uint32_t* ws_ptr, uint32_t* out, const Geom& g, const Tables& t,
+                        float* margin, float2* sspec_out, uint32_t* ws_out, hipStream_t s);
+
+}  // namespace pa

@@ -1,0 +1,235 @@
+"""ctypes front of libpa_hip.so (the C ABI of include/pa_engine.h).
+
+The Engine class mirrors the reference's per-block contract (mainloop's inputs and output,
+PrivacyAmplification.cu:1686-2103): a key of n+1 bits (n/32+1 RefBits words; h/32+1 words without
+key-rest XOR), an n-bit Toeplitz seed (n/32 words), vertical_len m and the two per-block flags
+do_xor_key_rest / do_compress.  The output is the amp_out byte string the reference sends on its
+amp-out socket (m/8 bytes, or n/16 bytes without compression).
+
+There is no fallback: if the HIP library is missing or no GPU is present, calls raise.
+"""
+import ctypes
+import os
+import pathlib
+import subprocess
+
+import numpy as np
+
+XOR_KEY_REST = 1
+COMPRESS = 2
+DEFAULT_FLAGS = XOR_KEY_REST | COMPRESS
+
+_PKG = pathlib.Path(__file__).resolve().parent
+_LIB = _PKG / "libpa_hip.so"
+_lib = None
+
+_STATUS = {
+    0: "ok", -1: "invalid argument", -2: "block size outside the supported family",
+    -3: "HIP runtime error", -4: "device memory or HBM budget exhausted", -5: "timed out",
+    -6: "unknown seed or ticket", -7: "unsupported request",
+}
+
+
+class EngineError(RuntimeError):
+    def __init__(self, status, what=""):
+        self.status = status
+        msg = _lib.pa_strerror(status).decode() if _lib is not None else _STATUS.get(status, "?")
+        super().__init__(f"{what}: {msg} (status {status})" if what else msg)
+
+
+class PaConfig(ctypes.Structure):
+    _fields_ = [
+        ("device", ctypes.c_int32),
+        ("max_n_log2", ctypes.c_uint32),
+        ("reduction_exp", ctypes.c_uint32),
+        ("pre_mul_reduction_exp", ctypes.c_uint32),
+        ("hbm_budget_bytes", ctypes.c_uint64),
+        ("out_reverse_endian", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
+def library_path():
+    return _LIB
+
+
+def build(force=False):
+    """Compile libpa_hip.so for gfx950 with the Makefile in csrc/ (hipcc cross-compiles; no GPU needed)."""
+    cmd = ["make", "-C", str(_PKG / "csrc")]
+    if force:
+        cmd.append("-B")
+    subprocess.run(cmd, check=True)
+    return _LIB
+
+
+def load_library():
+    """Load the in-tree HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not _LIB.exists():
+        raise RuntimeError(f"{_LIB} is missing: build it with privacyamplification_amd.build() "
+                           "(there is no CPU fallback)")
+    L = ctypes.CDLL(str(_LIB))
+    u32p, u8p, vp = ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint8), ctypes.c_void_p
+    eng = ctypes.c_void_p
+    L.pa_config_default.argtypes = [ctypes.POINTER(PaConfig)]
+    L.pa_create.argtypes = [ctypes.POINTER(PaConfig), ctypes.POINTER(eng)]
+    L.pa_destroy.argtypes = [eng]
+    L.pa_seed_put.argtypes = [eng, u32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32,
+                              ctypes.POINTER(ctypes.c_uint64)]
+    L.pa_seed_drop.argtypes = [eng, ctypes.c_uint64]
+    L.pa_run_block.argtypes = [eng, u32p, u32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, u8p]
+    L.pa_run_block_seeded.argtypes = [eng, u32p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                                      ctypes.c_uint32, u8p]
+    L.pa_submit.argtypes = [eng, u32p, u32p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                            ctypes.c_uint32, u8p, ctypes.POINTER(ctypes.c_uint64)]
+    L.pa_wait.argtypes = [eng, ctypes.c_uint64, ctypes.c_int64]
+    L.pa_block_device.argtypes = [eng, vp, vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, vp,
+                                  vp, vp]
+    L.pa_block_device_seeded.argtypes = [eng, vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                                         ctypes.c_uint32, vp, vp, vp]
+    L.pa_stage_timing.argtypes = [eng, ctypes.c_int]
+    L.pa_stage_times.argtypes = [eng, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_double),
+                                 ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]
+    L.pa_workspace_bytes.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
+    L.pa_workspace_bytes.restype = ctypes.c_uint64
+    L.pa_synth_bits.argtypes = [ctypes.c_uint64, ctypes.c_double, ctypes.c_uint64, u32p]
+    L.pa_strerror.argtypes = [ctypes.c_int]
+    L.pa_strerror.restype = ctypes.c_char_p
+    L.pa_version.restype = ctypes.c_char_p
+    for name in ("pa_config_default", "pa_create", "pa_destroy", "pa_seed_put", "pa_seed_drop",
+                 "pa_run_block", "pa_run_block_seeded", "pa_submit", "pa_wait", "pa_block_device",
+                 "pa_block_device_seeded", "pa_synth_bits", "pa_stage_timing", "pa_stage_times"):
+        getattr(L, name).restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def _check(status, what):
+    if status != 0:
+        raise EngineError(status, what)
+
+
+def _u32(a):
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+
+
+def synth_bits(seed, nbits, p=0.5):
+    """Synthetic RefBits (xoshiro256**, Bernoulli(p)) from the product library's generator."""
+    L = load_library()
+    out = np.zeros((nbits + 31) // 32, dtype=np.uint32)
+    _check(L.pa_synth_bits(seed, p, nbits, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))),
+           "pa_synth_bits")
+    return out
+
+
+def workspace_bytes(n, m, flags=DEFAULT_FLAGS):
+    return int(load_library().pa_workspace_bytes(n, m, flags))
+
+
+def out_bytes(n, m, do_compress=True):
+    return m // 8 if do_compress else n // 16
+
+
+class Engine:
+    """One engine per GPU (pa_create / pa_destroy)."""
+
+    def __init__(self, device=0, max_n_log2=27, hbm_budget_bytes=64 << 30, out_reverse_endian=True):
+        L = load_library()
+        cfg = PaConfig()
+        L.pa_config_default(ctypes.byref(cfg))
+        cfg.device, cfg.max_n_log2 = device, max_n_log2
+        cfg.hbm_budget_bytes = hbm_budget_bytes
+        cfg.out_reverse_endian = 1 if out_reverse_endian else 0
+        self._h = ctypes.c_void_p()
+        _check(L.pa_create(ctypes.byref(cfg), ctypes.byref(self._h)), "pa_create")
+        self._L = L
+
+    def close(self):
+        if self._h:
+            self._L.pa_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def flags(do_xor_key_rest=True, do_compress=True):
+        return (XOR_KEY_REST if do_xor_key_rest else 0) | (COMPRESS if do_compress else 0)
+
+    def amplify(self, key, seed, n, m, do_xor_key_rest=True, do_compress=True):
+        """One block, host buffers in and out; returns the amp_out bytes as a uint8 array."""
+        out = np.zeros(out_bytes(n, m, do_compress) + 8, dtype=np.uint8)
+        k, kp = _u32(key)
+        s, sp = _u32(seed)
+        _check(self._L.pa_run_block(self._h, kp, sp, n, m, self.flags(do_xor_key_rest, do_compress),
+                                    out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))), "pa_run_block")
+        return out[: out_bytes(n, m, do_compress)]
+
+    def seed_put(self, seed, n, m, reuse=-1):
+        s, sp = _u32(seed)
+        sid = ctypes.c_uint64()
+        _check(self._L.pa_seed_put(self._h, sp, n, m, reuse, ctypes.byref(sid)), "pa_seed_put")
+        return sid.value
+
+    def seed_drop(self, sid):
+        _check(self._L.pa_seed_drop(self._h, sid), "pa_seed_drop")
+
+    def amplify_seeded(self, key, sid, n, m, do_xor_key_rest=True, do_compress=True):
+        out = np.zeros(out_bytes(n, m, do_compress) + 8, dtype=np.uint8)
+        k, kp = _u32(key)
+        _check(self._L.pa_run_block_seeded(self._h, kp, sid, n, m,
+                                           self.flags(do_xor_key_rest, do_compress),
+                                           out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))),
+               "pa_run_block_seeded")
+        return out[: out_bytes(n, m, do_compress)]
+
+    def submit(self, key, seed, n, m, do_xor_key_rest=True, do_compress=True, sid=0):
+        """Asynchronous block; returns (ticket, out, keepalive).  Call wait(ticket) before reading out."""
+        out = np.zeros(out_bytes(n, m, do_compress) + 8, dtype=np.uint8)
+        k, kp = _u32(key)
+        if seed is not None:
+            s, sp = _u32(seed)
+        else:
+            s, sp = None, None
+        t = ctypes.c_uint64()
+        _check(self._L.pa_submit(self._h, kp, sp, sid, n, m, self.flags(do_xor_key_rest, do_compress),
+                                 out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), ctypes.byref(t)),
+               "pa_submit")
+        return t.value, out, (k, s)
+
+    def wait(self, ticket, timeout_ns=-1):
+        _check(self._L.pa_wait(self._h, ticket, timeout_ns), "pa_wait")
+
+    def block_device(self, d_key, d_seed, n, m, flags, d_out, stream=None, d_margin=None):
+        """Device-resident block: integer device addresses (e.g. torch tensor data_ptr())."""
+        _check(self._L.pa_block_device(self._h, d_key, d_seed, n, m, flags, d_out, stream, d_margin),
+               "pa_block_device")
+
+    def block_device_seeded(self, d_key, sid, n, m, flags, d_out, stream=None, d_margin=None):
+        _check(self._L.pa_block_device_seeded(self._h, d_key, sid, n, m, flags, d_out, stream, d_margin),
+               "pa_block_device_seeded")
+
+    def stage_timing(self, enable=True):
+        _check(self._L.pa_stage_timing(self._h, 1 if enable else 0), "pa_stage_timing")
+
+    def stage_times(self):
+        """{stage name: (total ms, launches)} since the last read (synchronises)."""
+        cap = 32
+        names = (ctypes.c_char_p * cap)()
+        ms = (ctypes.c_double * cap)()
+        cnt = (ctypes.c_uint32 * cap)()
+        k = ctypes.c_uint32(cap)
+        _check(self._L.pa_stage_times(self._h, names, ms, cnt, ctypes.byref(k)), "pa_stage_times")
+        return {names[i].decode(): (ms[i], cnt[i]) for i in range(k.value)}

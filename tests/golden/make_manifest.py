@@ -1,0 +1,49 @@
+"""Writes manifest.json: SHA3-256 of the oracle's amp_out for the seeded synthetic blocks the GPU
+tests and the bench use at sizes too large to recompute inside a test run.
+
+Inputs follow the measurement plan: xoshiro256** seeded per block, Bernoulli(0.5) bits, key of
+n+1 bits, seed of n bits, m = 3n/8, XOR and compress on.  Output digests come from the exact NTT
+oracle (oracle/pa_oracle.c, cross-checked against the direct GF(2) product at smaller n).
+Run: python tests/golden/make_manifest.py   (about 2 minutes on 8 cores)
+"""
+import hashlib
+import json
+import pathlib
+import sys
+import time
+
+ROOT = pathlib.Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT))
+from oracle import oracle as O  # noqa: E402
+
+
+def entry(name, n, m, key_seed, toeplitz_seed, p_key=0.5):
+    key = O.synth_bits(key_seed, n + 1, p_key)
+    seed = O.synth_bits(toeplitz_seed, n)
+    t = time.time()
+    out = O.amp_out(key, seed, n, m, method=O.NTT)
+    print(f"{name}: n=2^{n.bit_length()-1} {time.time()-t:.1f}s", flush=True)
+    return {"name": name, "n": n, "m": m, "key_seed": key_seed, "toeplitz_seed": toeplitz_seed,
+            "p_key": p_key, "do_xor_key_rest": True, "do_compress": True,
+            "sha3": hashlib.sha3_256(out.tobytes()).hexdigest(), "out_bytes": int(out.size)}
+
+
+def main():
+    O.build()
+    items = [entry("pr1_n20", 1 << 20, 3 << 17, 0x11, 0x22)]
+    n = 1 << 27
+    m = 3 * n // 8
+    for i in range(2):
+        items.append(entry(f"c2_dynamic_{i}", n, m, 0x1000 + i, 0x2000 + i))
+        items.append(entry(f"c2_static_{i}", n, m, 0x1000 + i, 0x5EED))
+    for e in (24, 26):
+        nn = 1 << e
+        for pk in (0.5, 0.9):
+            items.append(entry(f"c3_n{e}_p{pk}", nn, 3 * nn // 8, 0x3000 + e, 0x4000 + e, pk))
+    out = pathlib.Path(__file__).with_name("manifest.json")
+    out.write_text(json.dumps({"generator": "oracle NTT (exact)", "blocks": items}, indent=1) + "\n")
+    print("wrote", out)
+
+
+if __name__ == "__main__":
+    main()

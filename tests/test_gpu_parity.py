@@ -1,0 +1,146 @@
+"""Bit-exact parity of the HIP engine with the oracle and with the reference's golden vectors.
+
+Runs on an MI355X (marker gpu).  Every case prints its parameters before the GPU call."""
+import hashlib
+import json
+import pathlib
+
+import numpy as np
+import pytest
+
+from oracle import lbs
+
+pytestmark = pytest.mark.gpu
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+
+
+def sha3(b):
+    return hashlib.sha3_256(bytes(b)).hexdigest()
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+def test_lbs_golden_tiles(engine, lbs_fixture):
+    """G1/G2: the 60 tiles of LargeBlocksizeExample through the engine (n=2^11, m=1024, NoXOR,
+    no compression) reproduce every digest of Matrix.h."""
+    say("lbs tiles n=2048 noxor nocompress")
+    tile = lambda k, s: engine.amplify(k, s, 2048, 1024, do_xor_key_rest=False, do_compress=False)
+    chunks, inter, pre, kr, res, _ = lbs.run_schedule(lbs_fixture, tile)
+    assert chunks == lbs_fixture["chunk_sha3"]
+    assert inter == lbs_fixture["intermediate_sha3"]
+    assert (pre, kr, res) == (lbs_fixture["pre_xor_sha3"], lbs_fixture["key_rest_sha3"],
+                              lbs_fixture["result_sha3"])
+
+
+def test_lbs_golden_single_block(engine, lbs_fixture):
+    say("lbs single xor block n=16384")
+    key, seed, n, m = lbs.full_block_inputs(lbs_fixture)
+    assert sha3(engine.amplify(key, seed, n, m)) == lbs_fixture["result_sha3"]
+
+
+CASES = [(e, frac) for e in range(10, 21) for frac in ((3, 8),)] + \
+        [(e, frac) for e in (11, 14, 15, 17) for frac in ((1, 8), (1, 4), (1, 2))]
+
+
+@pytest.mark.parametrize("e,frac", CASES)
+@pytest.mark.parametrize("do_xor,do_compress", [(True, True), (False, True), (False, False), (True, False)])
+def test_random_blocks_vs_oracle(engine, oracle, e, frac, do_xor, do_compress):
+    n = 1 << e
+    m = n * frac[0] // frac[1]
+    say(f"random n=2^{e} m={m} xor={do_xor} compress={do_compress}")
+    key = oracle.synth_bits(0x100 + e, n + 1)
+    seed = oracle.synth_bits(0x200 + e, n)
+    if not do_xor:
+        key = key[: (n - m) // 32 + 1].copy()
+    ref = oracle.amp_out(key, seed, n, m, do_xor, do_compress, method=oracle.NTT)
+    got = engine.amplify(key, seed, n, m, do_xor, do_compress)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("e", [12, 14, 16, 20, 22])
+def test_static_seed_matches_dynamic(engine, oracle, e):
+    n, m = 1 << e, 3 * (1 << e) // 8
+    say(f"static seed n=2^{e}")
+    seed = oracle.synth_bits(0x5EED, n)
+    sid = engine.seed_put(seed, n, m)
+    try:
+        for i in range(3):
+            key = oracle.synth_bits(0x1000 + i, n + 1)
+            ref = oracle.amp_out(key, seed, n, m, method=oracle.NTT)
+            assert np.array_equal(engine.amplify_seeded(key, sid, n, m), ref)
+            assert np.array_equal(engine.amplify(key, seed, n, m), ref)
+    finally:
+        engine.seed_drop(sid)
+
+
+@pytest.mark.parametrize("p_key", [0.9, 0.1, 1.0, 0.0])
+def test_biased_keys(engine, oracle, p_key):
+    n, m = 1 << 18, 3 << 15
+    say(f"biased key p={p_key} n=2^18")
+    key = oracle.synth_bits(0x77, n + 1, p_key)
+    seed = oracle.synth_bits(0x78, n)
+    ref = oracle.amp_out(key, seed, n, m, method=oracle.NTT)
+    assert np.array_equal(engine.amplify(key, seed, n, m), ref)
+
+
+@pytest.mark.parametrize("n,m", [(3 << 15, 3 << 13), (5 << 12, 1 << 12), (100000 - 100000 % 32, 25024)])
+def test_non_power_of_two_blocks(engine, oracle, n, m):
+    """Blocks of n = 32k bits run on a power-of-two transform with the seed rotated by m
+    (same Toeplitz product y[p] = XOR_j x[j] s[(p-j) mod n])."""
+    say(f"non-2^k n={n} m={m}")
+    key = oracle.synth_bits(n, n + 1)
+    seed = oracle.synth_bits(n + 1, n)
+    ref = oracle.amp_out(key, seed, n, m, method=oracle.DIRECT)
+    assert np.array_equal(engine.amplify(key, seed, n, m), ref)
+
+
+def test_async_submit_wait(engine, oracle):
+    n, m = 1 << 16, 3 << 13
+    say("async submit x4")
+    jobs = []
+    for i in range(4):
+        key = oracle.synth_bits(0x900 + i, n + 1)
+        seed = oracle.synth_bits(0xA00 + i, n)
+        t, out, keep = engine.submit(key, seed, n, m)
+        jobs.append((t, out, keep, oracle.amp_out(key, seed, n, m, method=oracle.NTT)))
+    for t, out, keep, ref in jobs:
+        engine.wait(t)
+        assert np.array_equal(out[: ref.size], ref)
+
+
+def test_refbits_output_order(oracle):
+    import privacyamplification_amd as P
+    n, m = 1 << 15, 3 << 12
+    say("reverse_endian=0 engine")
+    key = oracle.synth_bits(1, n + 1)
+    seed = oracle.synth_bits(2, n)
+    with P.Engine(out_reverse_endian=False) as eng:
+        got = eng.amplify(key, seed, n, m)
+    assert np.array_equal(got, oracle.amp_out(key, seed, n, m, reverse_endian=False, method=oracle.NTT))
+
+
+def _manifest():
+    return json.loads((ROOT / "tests/golden/manifest.json").read_text())["blocks"]
+
+
+@pytest.mark.parametrize("item", _manifest(), ids=lambda it: it["name"])
+def test_golden_manifest(engine, oracle, item):
+    """Full-size blocks (2^20 .. 2^27) against the committed oracle digests."""
+    n, m = item["n"], item["m"]
+    say(f"manifest {item['name']} n={n}")
+    key = oracle.synth_bits(item["key_seed"], n + 1, item["p_key"])
+    seed = oracle.synth_bits(item["toeplitz_seed"], n)
+    assert sha3(engine.amplify(key, seed, n, m)) == item["sha3"]
+
+
+def test_linearity_at_full_size(engine, oracle):
+    """amp_out is GF(2)-linear in the key (T x and the key rest both are): out(a^b) = out(a)^out(b)."""
+    n, m = 1 << 27, 3 << 24
+    say("linearity n=2^27")
+    seed = oracle.synth_bits(0x2000, n)
+    a = oracle.synth_bits(0xA, n + 1)
+    b = oracle.synth_bits(0xB, n + 1)
+    oa, ob = engine.amplify(a, seed, n, m), engine.amplify(b, seed, n, m)
+    assert np.array_equal(engine.amplify(a ^ b, seed, n, m), oa ^ ob)
