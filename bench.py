@@ -74,6 +74,8 @@ def main():
     ap.add_argument("--log2n", type=int, default=27)
     ap.add_argument("--mode", choices=("dynamic", "static"), default="dynamic")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 512 MiB device copy (reference point for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (blocks are independent: "
@@ -97,7 +99,9 @@ def main():
     out = torch.empty(m // 8 + 16, dtype=torch.uint8, device=dev)
     margin = torch.zeros(1, dtype=torch.float32, device=dev)
     eng = P.Engine(device=0)
-    stream = torch.cuda.current_stream()
+    # a dedicated stream: the engine treats a null stream handle as its own internal stream
+    stream = torch.cuda.Stream(device=dev)
+    torch.cuda.set_stream(stream)
     sptr = stream.cuda_stream
     sids = []
     if args.mode == "static":
@@ -111,6 +115,14 @@ def main():
         else:
             eng.block_device(keys[j].data_ptr(), seeds[j].data_ptr(), n, m, flags, out.data_ptr(), sptr,
                              mg.data_ptr() if mg is not None else None)
+
+    if args.copy_calibration:
+        src = torch.empty(1 << 27, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     for i in range(args.warmup):
         step(i)

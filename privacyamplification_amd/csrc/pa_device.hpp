@@ -79,8 +79,11 @@ __device__ __forceinline__ void dft(float2 (&x)[R]) {
     }
 }
 
-// LDS index with one padding slot per 16 complex values (breaks power-of-two strides).
-__host__ __device__ constexpr int padi(int e) { return e + (e >> 4); }
+// LDS index with one padding slot per 32 complex values.  With 8-byte accesses over 64 banks this
+// keeps the Stockham patterns conflict-free: contiguous runs, first-stage scatters j*R + r for
+// R = 16 and 32, and runs of 16 lanes 256 apart all spread over both halves of the banks.
+constexpr int PAD_SHIFT = 5;
+__host__ __device__ constexpr int padi(int e) { return e + (e >> PAD_SHIFT); }
 // Stride between sequences stored back to back: padded length plus one odd slot, so that lanes
 // walking across sequences at a fixed position hit distinct banks.
 __host__ __device__ constexpr int seq_stride(int len) { return padi(len) + 1; }
@@ -100,30 +103,71 @@ __device__ __forceinline__ const T* opaque_ptr(const T* p) {
     return reinterpret_cast<const T*>(v);
 }
 
+// exp(-2 pi i y / 2^14), y in [0, 2^14), from a two-level table kept in LDS:
+// twl[0..128) = w^i, twl[128..256) = w^(128 i).  Two LDS reads and one product (about 1.5 ulp).
+constexpr int TWL_SIZE = 256;
+__device__ __forceinline__ float2 tw_lds(const float2* twl, int y) {
+    return cmul(twl[y & 127], twl[128 + ((y >> 7) & 127)]);
+}
+// Fills the LDS table from the global tw14 table (all threads of the workgroup take part; the
+// caller synchronises before first use).
+__device__ __forceinline__ void load_twl(float2* twl, const float2* __restrict__ tw14, int nthreads) {
+    for (int i = threadIdx.x; i < TWL_SIZE; i += nthreads)
+        twl[i] = i < 128 ? tw14[i] : tw14[((i - 128) * 128) & (TW14 - 1)];
+}
+
+// tw[r] = exp(-2 pi i r x / 2^14) for r < R from log2(R) table reads (r = 2^b) and at most
+// log2(R)-1 products for the other r.
+template <int R>
+__device__ __forceinline__ void twiddle_powers(float2 (&tw)[R], const float2* twl, int x) {
+    tw[0] = make_float2(1.f, 0.f);
+#pragma unroll
+    for (int b = 1; b < R; b <<= 1) tw[b] = tw_lds(twl, (b * x) & (TW14 - 1));
+#pragma unroll
+    for (int r = 3; r < R; ++r) {
+        if ((r & (r - 1)) == 0) continue;
+        const int hb = 1 << (31 - __builtin_clz(r));
+        tw[r] = cmul(tw[hb], tw[r - hb]);
+    }
+}
+
+// Hides an integer's value from the optimiser (same purpose as opaque_ptr, for per-lane
+// indices: address arithmetic repeated in two phases is otherwise computed once and kept live).
+__device__ __forceinline__ int opaque_int(int x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
 // One in-place Stockham radix-R stage over NSEQ sequences of length 2^LG in LDS.
 // Reads all butterflies' inputs, barrier, writes, barrier.
 template <int R, int NS, int LG, int NSEQ, int SEQS, int NT>
-__device__ __forceinline__ void stockham_stage(float2* buf, const float2* __restrict__ tw14_arg) {
-    const float2* __restrict__ tw14 = opaque_ptr(tw14_arg);
+__device__ __forceinline__ void stockham_stage(float2* buf, const float2* twl) {
     constexpr int L = 1 << LG;
     constexpr int BPS = L / R;
     constexpr int NB = NSEQ * BPS;
     constexpr int PER = (NB + NT - 1) / NT;
     float2 v[PER][R];
-    const int tid = threadIdx.x;
+    const int tid = opaque_int(threadIdx.x);  // per-stage address arithmetic, never kept live
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
         const int b = tid + i * NT;
         if ((NB % NT) == 0 || b < NB) {
             const int seq = b / BPS, j = b % BPS;
             const float2* base = buf + seq * SEQS;
+            if constexpr (BPS % (1 << PAD_SHIFT) == 0) {
+                // padi(j + r*BPS) = padi(j) + r*padi(BPS): one address, immediate offsets
+                const float2* rb = base + padi(j);
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[i][r] = base[padi(j + r * BPS)];
+                for (int r = 0; r < R; ++r) v[i][r] = rb[r * padi(BPS)];
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[i][r] = base[padi(j + r * BPS)];
+            }
             if constexpr (NS > 1) {
-                const int jj = j % NS;
-                constexpr int STEP = TW14 / (NS * R);
+                float2 tw[R];
+                twiddle_powers<R>(tw, twl, (j % NS) * (TW14 / (NS * R)));
 #pragma unroll
-                for (int r = 1; r < R; ++r) v[i][r] = cmul(v[i][r], tw14[(r * jj) * STEP]);
+                for (int r = 1; r < R; ++r) v[i][r] = cmul(v[i][r], tw[r]);
             }
             dft<R>(v[i]);
         }
@@ -136,33 +180,76 @@ __device__ __forceinline__ void stockham_stage(float2* buf, const float2* __rest
             const int seq = b / BPS, j = b % BPS;
             float2* base = buf + seq * SEQS;
             const int d = (j / NS) * NS * R + (j % NS);
+            if constexpr (NS % (1 << PAD_SHIFT) == 0) {
+                float2* wb = base + padi(d);
 #pragma unroll
-            for (int r = 0; r < R; ++r) base[padi(d + r * NS)] = v[i][r];
+                for (int r = 0; r < R; ++r) wb[r * padi(NS)] = v[i][r];
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; ++r) base[padi(d + r * NS)] = v[i][r];
+            }
         }
     }
     __syncthreads();
 }
 
+// First Stockham stage (span 1) of a length-L transform for one butterfly j whose inputs
+// v[r] = x[j + r L/R] are already in registers: DFT, then the outputs go to LDS at j R + r.
+template <int R>
+__device__ __forceinline__ void stage_first_to_lds(float2 (&v)[R], float2* seqbuf, int j) {
+    dft<R>(v);
+#pragma unroll
+    for (int r = 0; r < R; ++r) seqbuf[padi(j * R + r)] = v[r];
+}
+
+// Last Stockham stage (span L/R) for one butterfly j, read from LDS; the outputs stay in
+// registers: v[r] = X[j + r L/R].
+template <int R, int L>
+__device__ __forceinline__ void stage_last_from_lds(float2 (&v)[R], const float2* seqbuf, int j,
+                                                    const float2* twl) {
+    constexpr int NS = L / R;
+    if constexpr (NS % (1 << PAD_SHIFT) == 0) {
+        const float2* rb = seqbuf + padi(j);
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = rb[r * padi(NS)];
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = seqbuf[padi(j + r * NS)];
+    }
+    float2 tw[R];
+    twiddle_powers<R>(tw, twl, j * (TW14 / L));
+#pragma unroll
+    for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw[r]);
+    dft<R>(v);
+}
+
+// Radix-16 stages from span NS up to 2^LG; a final radix-32 stage absorbs a leftover factor 2.
 template <int NS, int LG, int NSEQ, int SEQS, int NT>
-__device__ __forceinline__ void stockham_rest(float2* buf, const float2* __restrict__ tw14) {
-    if constexpr (NS < (1 << LG)) {
-        stockham_stage<16, NS, LG, NSEQ, SEQS, NT>(buf, tw14);
-        stockham_rest<NS * 16, LG, NSEQ, SEQS, NT>(buf, tw14);
+__device__ __forceinline__ void stockham_rest(float2* buf, const float2* twl) {
+    constexpr int L = 1 << LG;
+    if constexpr (NS * 32 == L) {
+        stockham_stage<32, NS, LG, NSEQ, SEQS, NT>(buf, twl);
+    } else if constexpr (NS < L) {
+        stockham_stage<16, NS, LG, NSEQ, SEQS, NT>(buf, twl);
+        stockham_rest<NS * 16, LG, NSEQ, SEQS, NT>(buf, twl);
     }
 }
 
 // Forward FFT of NSEQ sequences of length 2^LG (LG <= 14) held in LDS at buf[s*SEQS + padi(e)].
 // Natural order in and out.  The caller must have synchronised after filling buf; the routine
-// ends with a barrier.
-template <int LG, int NSEQ, int SEQS, int NT>
-__device__ __forceinline__ void lds_fft(float2* buf, const float2* __restrict__ tw14) {
+// ends with a barrier.  twl: the LDS twiddle table (tw_lds).  Plan: radix-16 stages, a leading radix-2/4/8 for LG = 2,3 (mod 4), a
+// closing radix-32 for LG = 1 (mod 4), LG >= 5.
+template <int LG, int NSEQ, int SEQS, int NT, bool RADIX32 = true>
+__device__ __forceinline__ void lds_fft(float2* buf, const float2* twl) {
     static_assert(LG >= 1 && LG <= TW14_LOG, "FFT length out of range");
-    constexpr int R0 = 1 << (LG % 4);
-    if constexpr (R0 > 1) {
-        stockham_stage<R0, 1, LG, NSEQ, SEQS, NT>(buf, tw14);
-        stockham_rest<R0, LG, NSEQ, SEQS, NT>(buf, tw14);
+    constexpr int REM = LG % 4;
+    if constexpr (REM == 1 && LG >= 5 && RADIX32) {
+        stockham_rest<1, LG, NSEQ, SEQS, NT>(buf, twl);
+    } else if constexpr (REM != 0) {
+        stockham_stage<(1 << REM), 1, LG, NSEQ, SEQS, NT>(buf, twl);
+        stockham_rest<(1 << REM), LG, NSEQ, SEQS, NT>(buf, twl);
     } else {
-        stockham_rest<1, LG, NSEQ, SEQS, NT>(buf, tw14);
+        stockham_rest<1, LG, NSEQ, SEQS, NT>(buf, twl);
     }
 }
 

@@ -106,7 +106,7 @@ struct Workspace {
     uint32_t* tbk = nullptr;   // N/D words
     uint32_t* tbs = nullptr;
     uint8_t* nib = nullptr;    // L/4 bytes
-    uint32_t* counts = nullptr;  // [0] key popcount, [1] seed popcount
+    uint32_t* counts = nullptr;  // POP_SLOTS key popcount partials, then POP_SLOTS seed partials
     uint32_t* d_key = nullptr;   // staging for the host-buffer API
     uint32_t* d_seed = nullptr;
     uint32_t* d_out = nullptr;
@@ -116,7 +116,7 @@ struct Workspace {
 struct Seed {
     Geom g{};
     float2* spec = nullptr;    // N+1 complex split spectrum
-    uint32_t* ws = nullptr;    // device popcount
+    uint32_t* ws = nullptr;    // device popcount (POP_SLOTS partials)
     int32_t reuse = 0;
     uint64_t bytes = 0;
 };
@@ -136,6 +136,7 @@ struct pa_engine {
     pa_config cfg{};
     hipStream_t stream = nullptr;
     float2* tw14 = nullptr;
+    int cus = 256;  // compute units of the device (persistent kernels launch one workgroup each)
     std::map<uint32_t, std::unique_ptr<Workspace>> ws;
     std::map<pa_seed_id, Seed> seeds;
     std::map<pa_ticket, Ticket> tickets;
@@ -207,7 +208,7 @@ int get_workspace(pa_engine* e, const Geom& g, Workspace** out) {
     std::vector<A> allocs = {
         {(void**)&w->lo, 16384 * sizeof(float2)},
         {(void**)&w->hi, hi_n * sizeof(float2)},
-        {(void**)&w->counts, 256},
+        {(void**)&w->counts, 2 * pa::POP_SLOTS * sizeof(uint32_t)},
         {(void**)&w->d_key, key_bytes},
         {(void**)&w->d_seed, key_bytes},
         {(void**)&w->d_out, out_bytes},
@@ -283,19 +284,21 @@ int run_device(pa_engine* e, const Geom& g, Workspace* w, const uint32_t* key, c
         }));
         return PA_OK;
     }
-    PA_HIP(timed(e, ST_ZERO, s, [&] { return pa::launch_zero_u32(w->counts, 2, s); }));
-    PA_HIP(timed(e, ST_TRANSPOSE_KEY, s, [&] { return pa::launch_transpose(key, w->tbk, w->counts + 0, g, 0, s); }));
+    uint32_t* key_cnt = w->counts;
+    uint32_t* seed_cnt = w->counts + pa::POP_SLOTS;
+    PA_HIP(timed(e, ST_ZERO, s, [&] { return pa::launch_zero_u32(w->counts, 2 * pa::POP_SLOTS, s); }));
+    PA_HIP(timed(e, ST_TRANSPOSE_KEY, s, [&] { return pa::launch_transpose(key, w->tbk, key_cnt, g, 0, s); }));
     PA_HIP(timed(e, ST_FWD_COLS_KEY, s, [&] { return pa::launch_fwd_cols(w->tbk, w->Yk, g, t, s); }));
     if (!sspec) {
-        PA_HIP(timed(e, ST_TRANSPOSE_SEED, s, [&] { return pa::launch_transpose(seed, w->tbs, w->counts + 1, g, 1, s); }));
+        PA_HIP(timed(e, ST_TRANSPOSE_SEED, s, [&] { return pa::launch_transpose(seed, w->tbs, seed_cnt, g, 1, s); }));
         PA_HIP(timed(e, ST_FWD_COLS_SEED, s, [&] { return pa::launch_fwd_cols(w->tbs, w->Ys, g, t, s); }));
-        PA_HIP(timed(e, ST_ROWS_SEEDPREP, s, [&] { return pa::launch_rows(pa::ROWS_SEEDPREP, w->Yk, w->Ys, g, t, s); }));
-        sspec = w->Ys;
-        ws_dev = w->counts + 1;
+        PA_HIP(timed(e, ST_ROWS, s, [&] { return pa::launch_rows(pa::ROWS_DYN, w->Yk, w->Ys, g, t, e->cus, s); }));
+        ws_dev = seed_cnt;
+    } else {
+        float2* S = const_cast<float2*>(sspec);
+        PA_HIP(timed(e, ST_ROWS, s, [&] { return pa::launch_rows(pa::ROWS_STATIC, w->Yk, S, g, t, e->cus, s); }));
     }
-    float2* S = const_cast<float2*>(sspec);
-    PA_HIP(timed(e, ST_ROWS, s, [&] { return pa::launch_rows(pa::ROWS_STATIC, w->Yk, S, g, t, s); }));
-    PA_HIP(timed(e, ST_INV_COLS, s, [&] { return pa::launch_inv_cols(w->Yk, w->nib, g, w->counts + 0, ws_dev, t, margin, s); }));
+    PA_HIP(timed(e, ST_INV_COLS, s, [&] { return pa::launch_inv_cols(w->Yk, w->nib, g, key_cnt, ws_dev, t, margin, s); }));
     PA_HIP(timed(e, ST_PACK, s, [&] { return pa::launch_pack(w->nib, key, out, g, s); }));
     return PA_OK;
 }
@@ -331,6 +334,7 @@ int pa_create(const pa_config* cfg, pa_engine** out) {
     if (e->cfg.max_n_log2 == 0) e->cfg.max_n_log2 = 27;
     PA_HIP(hipSetDevice(e->cfg.device));
     PA_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    PA_HIP(hipDeviceGetAttribute(&e->cus, hipDeviceAttributeMultiprocessorCount, e->cfg.device));
     int rc = dev_alloc(e.get(), (void**)&e->tw14, 16384 * sizeof(float2));
     if (rc != PA_OK) return rc;
     auto tw = twiddles(16384, 16384, 1);
@@ -386,10 +390,10 @@ int pa_seed_put(pa_engine* e, const uint32_t* seed, uint32_t n, uint32_t m, int3
     Seed sd;
     sd.g = g;
     sd.reuse = reuse;
-    sd.bytes = (uint64_t)(g.N + 1) * sizeof(float2) + 256;
+    sd.bytes = (uint64_t)(g.N + 1) * sizeof(float2) + pa::POP_SLOTS * sizeof(uint32_t);
     rc = dev_alloc(e, (void**)&sd.spec, (size_t)(g.N + 1) * sizeof(float2));
     if (rc != PA_OK) return rc;
-    rc = dev_alloc(e, (void**)&sd.ws, 256);
+    rc = dev_alloc(e, (void**)&sd.ws, pa::POP_SLOTS * sizeof(uint32_t));
     if (rc != PA_OK) {
         dev_free(e, sd.spec, (size_t)(g.N + 1) * sizeof(float2));
         return rc;
@@ -400,10 +404,10 @@ int pa_seed_put(pa_engine* e, const uint32_t* seed, uint32_t n, uint32_t m, int3
     if (g.N1 == 1) {
         PA_HIP(pa::launch_small(nullptr, w->d_seed, nullptr, nullptr, nullptr, g, t, nullptr, sd.spec, sd.ws, s));
     } else {
-        PA_HIP(pa::launch_zero_u32(sd.ws, 1, s));
+        PA_HIP(pa::launch_zero_u32(sd.ws, pa::POP_SLOTS, s));
         PA_HIP(pa::launch_transpose(w->d_seed, w->tbs, sd.ws, g, 1, s));
         PA_HIP(pa::launch_fwd_cols(w->tbs, sd.spec, g, t, s));
-        PA_HIP(pa::launch_rows(pa::ROWS_SEEDPREP, nullptr, sd.spec, g, t, s));
+        PA_HIP(pa::launch_rows(pa::ROWS_SEEDPREP, nullptr, sd.spec, g, t, e->cus, s));
     }
     PA_HIP(hipStreamSynchronize(s));
     *id = e->next_seed++;
@@ -418,7 +422,7 @@ int pa_seed_drop(pa_engine* e, pa_seed_id id) {
     if (it == e->seeds.end()) return PA_ERR_NOT_FOUND;
     (void)hipStreamSynchronize(e->stream);
     dev_free(e, it->second.spec, (size_t)(it->second.g.N + 1) * sizeof(float2));
-    dev_free(e, it->second.ws, 256);
+    dev_free(e, it->second.ws, pa::POP_SLOTS * sizeof(uint32_t));
     e->seeds.erase(it);
     return PA_OK;
 }

@@ -42,6 +42,14 @@ __device__ __forceinline__ uint32_t seed_word(const uint32_t* __restrict__ seed,
     return seed[s];
 }
 
+// Popcounts are accumulated in POP_SLOTS partial counters (spread atomics); readers sum them.
+__device__ __forceinline__ uint32_t pop_total(const uint32_t* __restrict__ c) {
+    uint32_t t = 0;
+#pragma unroll 8
+    for (int i = 0; i < POP_SLOTS; ++i) t += c[i];
+    return t;
+}
+
 // corr = (wk * ws / L) mod 2, exactly: the mean term that zeroing both DC bins removes
 // (calculateCorrectionFloat, PrivacyAmplification.cu:363-373, evaluated in 64-bit integers).
 __device__ __forceinline__ float correction(uint32_t wk, uint32_t ws, uint32_t L) {
@@ -56,217 +64,467 @@ __device__ __forceinline__ uint32_t round_bit(float v, float& dist) {
     return (uint32_t)__float2int_rn(v) & 1u;
 }
 
-__global__ __launch_bounds__(64) void k_zero_u32(uint32_t* p, uint32_t count) {
-    const uint32_t i = threadIdx.x;
-    if (i < count) p[i] = 0u;
+__global__ __launch_bounds__(256) void k_zero_u32(uint32_t* p, uint32_t count) {
+    for (uint32_t i = threadIdx.x; i < count; i += 256) p[i] = 0u;
 }
 
 // ------------------------------------------------------------------ bit transpose ----
 // tb[u*N2 + t2] = E | O << 16 with E bit v = x[2(t2 + N2(u + M v))], O bit v = the odd bit.
+// Each thread writes TB_PER outputs; popcounts are reduced per workgroup into one of POP_SLOTS
+// counters.
 template <int D>
 __global__ __launch_bounds__(256) void k_transpose_bits(const uint32_t* __restrict__ bits,
                                                         uint32_t* __restrict__ tb,
                                                         uint32_t* __restrict__ count, Geom g,
                                                         int is_seed) {
+    __shared__ uint32_t red[4];
     const uint32_t total = g.M * g.N2;
-    const uint32_t idx = blockIdx.x * 256u + threadIdx.x;
     uint32_t pc = 0;
-    if (idx < total) {
-        const uint32_t t2 = idx & (g.N2 - 1u), u = idx >> g.lg2;
-        uint32_t E = 0, O = 0;
 #pragma unroll
-        for (int v = 0; v < D; ++v) {
-            const uint32_t pos = 2u * (t2 + g.N2 * (u + g.M * (uint32_t)v));
-            const uint32_t widx = pos >> 5, sh = pos & 31u;
-            const uint32_t w = is_seed ? seed_word(bits, g, widx) : key_word(bits, g, widx);
-            E |= ((w >> (31u - sh)) & 1u) << v;
-            O |= ((w >> (30u - sh)) & 1u) << v;
+    for (int q = 0; q < TB_PER; ++q) {
+        const uint32_t idx = (blockIdx.x * TB_PER + q) * 256u + threadIdx.x;
+        if (idx < total) {
+            const uint32_t t2 = idx & (g.N2 - 1u), u = idx >> g.lg2;
+            uint32_t E = 0, O = 0;
+#pragma unroll
+            for (int v = 0; v < D; ++v) {
+                const uint32_t pos = 2u * (t2 + g.N2 * (u + g.M * (uint32_t)v));
+                const uint32_t widx = pos >> 5, sh = pos & 31u;
+                const uint32_t w = is_seed ? seed_word(bits, g, widx) : key_word(bits, g, widx);
+                E |= ((w >> (31u - sh)) & 1u) << v;
+                O |= ((w >> (30u - sh)) & 1u) << v;
+            }
+            tb[idx] = E | (O << 16);
+            pc += (uint32_t)(__popc(E) + __popc(O));
         }
-        tb[idx] = E | (O << 16);
-        pc = (uint32_t)(__popc(E) + __popc(O));
     }
     for (int off = 32; off > 0; off >>= 1) pc += __shfl_xor(pc, off);
-    if ((threadIdx.x & 63u) == 0u && pc) atomicAdd(count, pc);
+    if ((threadIdx.x & 63u) == 0u) red[threadIdx.x >> 6] = pc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t t = red[0] + red[1] + red[2] + red[3];
+        if (t) atomicAdd(count + (blockIdx.x % POP_SLOTS), t);
+    }
 }
 
 // ------------------------------------------------------------------ pass A ----
 // Workgroup (column group grp of 16 columns, class c): computes Y[k1][t2] for k1 = c + D*k1',
 // t2 in the group:  Y = w_N^{t2 k1} * FFT_M( w_N1^{u c} * sum_v z[t2 + N2(u + M v)] w_D^{v c} ).
+// The class sum over v is four 16-entry table lookups per operand half (bits of tb).
+// Plan 16 x 32 for M = 512: stage 1 computed in registers from the bit masks, stage 2 written
+// straight to Y (thread t: column t mod 16, butterfly t / 16, rows k1' = j + 16 r).
 template <int LGM, int D>
-__global__ __launch_bounds__(256) void k_fwd_cols(const uint32_t* __restrict__ tb,
-                                                  float2* __restrict__ Y, Geom g,
-                                                  const float2* __restrict__ tw14,
-                                                  const float2* __restrict__ twlo,
-                                                  const float2* __restrict__ twhi) {
+__global__ __launch_bounds__(FWD_NT) void k_fwd_cols(const uint32_t* __restrict__ tb,
+                                                     float2* __restrict__ Y, Geom g,
+                                                     const float2* __restrict__ tw14,
+                                                     const float2* __restrict__ twlo,
+                                                     const float2* __restrict__ twhi) {
     constexpr int M = 1 << LGM;
-    constexpr int NT = 256;
+    constexpr int NT = FWD_NT;
     constexpr int SEQS = seq_stride(M);
     __shared__ float2 buf[16 * SEQS];
-    __shared__ float2 TE[256];
+    __shared__ float2 twl[TWL_SIZE];
+    constexpr int NQ = D < 4 ? 1 : D / 4;  // 4-bit tables: v in [4q, 4q+4)
+    __shared__ float2 TQ[NQ][16];         // TQ[q][nib] = sum_b bit_b(nib) w_D^{(4q+b) c}
+    __shared__ float2 TU[M];              // w_N1^{u c}
     const uint32_t logical = xcd_logical(blockIdx.x, gridDim.x);
     const uint32_t c = logical % D, grp = logical / D;
-    constexpr int VB = D < 8 ? D : 8;
-    for (int b = threadIdx.x; b < 256; b += NT) {
+    load_twl(twl, tw14, NT);
+    for (int e = threadIdx.x; e < NQ * 16; e += NT) {
+        const int q = e / 16, nib = e % 16;
         float2 acc = make_float2(0.f, 0.f);
-        for (int v = 0; v < VB; ++v)
-            if ((b >> v) & 1) acc = cadd(acc, tw14[((v * c) % D) * (TW14 / D)]);
-        TE[b] = acc;
+        for (int b = 0; b < 4; ++b) {
+            const int v = 4 * q + b;
+            if (v < D && ((nib >> b) & 1)) acc = cadd(acc, tw14[((v * c) % D) * (TW14 / D)]);
+        }
+        TQ[q][nib] = acc;
     }
-    const float2 w8c = (D > 8) ? tw14[((8u * c) % D) * (TW14 / D)] : make_float2(1.f, 0.f);
-    __syncthreads();
     const uint32_t step_n1 = TW14 / g.N1;
-    for (int e = threadIdx.x; e < 16 * M; e += NT) {
-        const int col = e & 15, u = e >> 4;
-        const uint32_t t = tb[(uint32_t)u * g.N2 + grp * 16u + (uint32_t)col];
-        const uint32_t E = t & 0xFFFFu, O = t >> 16;
-        float2 sE = TE[E & 255u], sO = TE[O & 255u];
-        if constexpr (D > 8) {
-            sE = cadd(sE, cmul(w8c, TE[E >> 8]));
-            sO = cadd(sO, cmul(w8c, TE[O >> 8]));
+    for (int u = threadIdx.x; u < M; u += NT) TU[u] = tw14[((uint32_t)u * c) * step_n1];
+    // the element z(col, u) of this class (both halves of tb: even and odd bits)
+    auto zval = [&](uint32_t t, int u) {
+        float2 sE = TQ[0][t & 15u], sO = TQ[0][(t >> 16) & 15u];
+#pragma unroll
+        for (int q = 1; q < NQ; ++q) {
+            sE = cadd(sE, TQ[q][(t >> (4 * q)) & 15u]);
+            sO = cadd(sO, TQ[q][(t >> (16 + 4 * q)) & 15u]);
         }
         float2 z = make_float2(sE.x - sO.y, sE.y + sO.x);
         if (c == 0) {
             z.x -= 0.5f * (float)D;
             z.y -= 0.5f * (float)D;
+        } else {
+            z = cmul(z, TU[u]);
         }
-        if (c != 0) z = cmul(z, tw14[((uint32_t)u * c) * step_n1]);
-        buf[col * SEQS + padi(u)] = z;
-    }
-    __syncthreads();
-    lds_fft<LGM, 16, SEQS, NT>(buf, tw14);
-    for (int e = threadIdx.x; e < 16 * M; e += NT) {
-        const int col = e & 15, k1p = e >> 4;
-        const uint32_t k1 = c + (uint32_t)D * (uint32_t)k1p, t2 = grp * 16u + (uint32_t)col;
-        float2 y = buf[col * SEQS + padi(k1p)];
-        y = cmul(y, tw_two_level(twlo, twhi, 2u * t2 * k1));
-        Y[(size_t)k1 * g.N2 + t2] = y;
+        return z;
+    };
+    if constexpr (LGM == 9) {
+        static_assert(NT == 256, "fused pass A expects 256 threads");
+        const int col = threadIdx.x & 15;
+        const uint32_t t2 = grp * 16u + (uint32_t)col;
+        // stage 1 (radix 16): butterflies j = t/16 + 16 i (i < 2), inputs u = j + 32 r
+        uint32_t tv[2][16];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int j = (threadIdx.x >> 4) + 16 * i;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) tv[i][r] = tb[(uint32_t)(j + 32 * r) * g.N2 + t2];
+        }
+        __syncthreads();  // tables
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int j = (threadIdx.x >> 4) + 16 * i;
+            float2 v[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) v[r] = zval(tv[i][r], j + 32 * r);
+            stage_first_to_lds<16>(v, buf + col * SEQS, j);
+        }
+        __syncthreads();
+        // stage 2 (radix 32, last): butterfly j = t/16, outputs k1' = j + 16 r
+        const int j = threadIdx.x >> 4;
+        float2 F[32];
+        stage_last_from_lds<32, M>(F, buf + col * SEQS, j, twl);
+        // w_N^{t2 k1} = w_L^{2 t2 (c + D j)} * w_L^{2 t2 16 D r}
+        const float2 base = tw_two_level(twlo, twhi, (uint32_t)((2ull * t2 * (c + (uint64_t)D * j)) & (g.L - 1u)));
+        const float2 step = tw_two_level(twlo, twhi, (uint32_t)((2ull * t2 * 16ull * D) & (g.L - 1u)));
+        float2 tw = base;
+#pragma unroll
+        for (int r = 0; r < 32; ++r) {
+            if ((r & 7) == 0 && r)  // re-anchor every 8 steps to bound the recurrence error
+                tw = cmul(base, tw_two_level(twlo, twhi, (uint32_t)((2ull * t2 * 16ull * D * r) & (g.L - 1u))));
+            const uint32_t k1 = c + (uint32_t)D * (uint32_t)(j + 16 * r);
+            Y[(size_t)k1 * g.N2 + t2] = cmul(F[r], tw);
+            tw = cmul(tw, step);
+        }
+    } else {
+        constexpr int IPT = (16 * M + NT - 1) / NT;
+        uint32_t tbv[IPT];
+#pragma unroll
+        for (int i = 0; i < IPT; ++i) {
+            const int e = threadIdx.x + i * NT;
+            tbv[i] = (e < 16 * M) ? tb[(uint32_t)(e >> 4) * g.N2 + grp * 16u + (uint32_t)(e & 15)] : 0u;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < IPT; ++i) {
+            const int e = threadIdx.x + i * NT;
+            if (e < 16 * M) buf[(e & 15) * SEQS + padi(e >> 4)] = zval(tbv[i], e >> 4);
+        }
+        __syncthreads();
+        lds_fft<LGM, 16, SEQS, NT>(buf, twl);
+        for (int e = threadIdx.x; e < 16 * M; e += NT) {
+            const int col = e & 15, k1p = e >> 4;
+            const uint32_t k1 = c + (uint32_t)D * (uint32_t)k1p, t2 = grp * 16u + (uint32_t)col;
+            const float2 tw = tw_two_level(twlo, twhi, (uint32_t)((2ull * t2 * k1) & (g.L - 1u)));
+            Y[(size_t)k1 * g.N2 + t2] = cmul(buf[col * SEQS + padi(k1p)], tw);
+        }
     }
 }
 
 // ------------------------------------------------------------------ pass B ----
-// Workgroup w owns rows (w, N1-w), or (0, N1/2) for w = 0: the rows whose frequencies pair
-// k <-> N-k in the real split.  After the forward row FFTs each pair (k, N-k) is one work unit:
-// real split of both operands, DC zeroing, product, inverse split, written back in place, so
-// nothing is held in registers across a barrier.
-//   ROWS_SEEDPREP: rows of the seed's pass-A output -> split seed spectrum S (in place, plus the
-//                  Nyquist value S[N] at index N).
-//   ROWS_STATIC:   rows of the key's pass-A output and the resident S -> R'[k1][t2] =
-//                  w_N^{t2 k1} * FFT_N2(conj Q)[t2], written in place over the key rows.
-// The split twiddle w_L^k with k = row + N1*k2 factors as w_L^row * w_{2 N2}^k2 = w_L^row * tw14[k2].
+// Item w (one row pair) owns rows (w, N1-w), or (0, N1/2) for w = 0: the rows whose frequencies
+// pair k <-> N-k in the real split (k = row + N1*k2).  Row transforms have N2 = 8192 points:
+//   forward plan 32 x 16 x 16: stage 1 straight from global memory, stage 3 into registers;
+//   inverse plan 16 x 16 x 32: stage 1 from registers, stage 3 straight to global memory.
+// After forward stage 3 a thread holds Z at k2 = jA + 512 r (row sA) and k2 = jB + 512 r (row sB)
+// for r < 16, chosen so that the partners of the first are the second read backwards:
+//   w != 0:                     (row0, t)      and (row1, 511-t)        pairs (A r, B 15-r)
+//   w == 0, 0 < t < 256:        (row0, t)      and (row0, 512-t)        pairs (A r, B 15-r)
+//   w == 0, t >= 256:           (row1, t-256)  and (row1, 767-t)        pairs (A r, B 15-r)
+//   w == 0, t == 0:             (row0, 0)      and (row0, 256)          pairs (A r, A 16-r) with
+//                               k = 0 (DC zeroed, Nyquist carried) and k = N/2 self-paired,
+//                               and (B r, B 15-r)
+// so the real split, product and inverse split need no data exchange, and their results are
+// exactly the inputs of inverse stage 1.
+//   ROWS_SEEDPREP: seed rows (pass-A output) -> split seed spectrum S, in place (+ S[N] at index N).
+//   ROWS_STATIC:   key rows + resident S -> R'[k1][t2] = w_N^{t2 k1} FFT_N2(conj Q)[t2], in place.
+//   ROWS_DYN:      key rows + seed rows; S is formed first and held in registers.
+// Persistent: each workgroup walks the items blockIdx.x, +gridDim.x, ...; the global inputs of
+// the next transform are requested while the current one runs (register prefetch).
+struct RowMap {
+    uint32_t row0, row1, rowA, rowB;
+    int sA, jA, sB, jB;
+    bool special;
+};
+__device__ __forceinline__ RowMap row_map(uint32_t w, uint32_t N1, int tid) {
+    RowMap m;
+    const bool w0 = (w == 0);
+    m.row0 = w;
+    m.row1 = w0 ? (N1 >> 1) : (N1 - w);
+    m.special = w0 && tid == 0;
+    if (!w0) { m.sA = 0; m.jA = tid; m.sB = 1; m.jB = 511 - tid; }
+    else if (tid == 0) { m.sA = 0; m.jA = 0; m.sB = 0; m.jB = 256; }
+    else if (tid < 256) { m.sA = 0; m.jA = tid; m.sB = 0; m.jB = 512 - tid; }
+    else { m.sA = 1; m.jA = tid - 256; m.sB = 1; m.jB = 767 - tid; }
+    m.rowA = m.sA ? m.row1 : m.row0;
+    m.rowB = m.sB ? m.row1 : m.row0;
+    return m;
+}
+
 template <int MODE>
-__global__ __launch_bounds__(ROWS_NT) void k_rows(float2* __restrict__ Yk,
-                                                  float2* __restrict__ Sst, Geom g,
-                                                  const float2* __restrict__ tw14,
+__global__ __launch_bounds__(ROWS_NT) void k_rows(float2* __restrict__ Yk, float2* __restrict__ Sst,
+                                                  Geom g, const float2* __restrict__ tw14,
                                                   const float2* __restrict__ twlo,
                                                   const float2* __restrict__ twhi) {
-    constexpr int LG2 = ROW_LOG, N2 = 1 << LG2, NT = ROWS_NT, SEQS = seq_stride(N2);
+    constexpr int N2 = 1 << ROW_LOG, NT = ROWS_NT, SEQS = seq_stride(N2);
+    static_assert(N2 == 8192 && NT == 512, "row pass is written for 8192-point rows, 512 threads");
     static_assert(2 * N2 == TW14, "row split twiddle assumes N2 = 2^13");
-    constexpr int PER = 2 * N2 / NT, HALF = PER / 2;
     __shared__ float2 buf[2 * SEQS];
-    const uint32_t w = blockIdx.x, N1 = g.N1;
-    const uint32_t rows[2] = {w, (w == 0) ? (N1 >> 1) : (N1 - w)};
+    __shared__ float2 twl[TWL_SIZE];
+    load_twl(twl, tw14, NT);
+    const uint32_t N1 = g.N1, items = N1 >> 1;
     const int tid = threadIdx.x;
-    const float2 wrow[2] = {tw_two_level(twlo, twhi, rows[0]), tw_two_level(twlo, twhi, rows[1])};
-    float2* __restrict__ data = (MODE == ROWS_SEEDPREP) ? Sst : Yk;
+    // stage-1 butterfly of this thread: row (t >> 8), j = t & 255, inputs j + 256 r
+    const int s1seq = tid >> 8, s1j = tid & 255;
 
+    auto s1_load = [&](const float2* __restrict__ src, uint32_t w, float2 (&v)[32]) {
+        const uint32_t row = s1seq ? ((w == 0) ? (N1 >> 1) : (N1 - w)) : w;
+        const float2* rp = src + (size_t)row * N2 + s1j;
 #pragma unroll
-    for (int i = 0; i < PER; ++i) {
-        const int seq = i / HALF, k2 = tid + (i % HALF) * NT;
-        buf[seq * SEQS + padi(k2)] = data[(size_t)rows[seq] * N2 + k2];
-    }
-    __syncthreads();
-    lds_fft<LG2, 2, SEQS, NT>(buf, tw14);
-
-    // One element pair: (seqA, kA2) holds frequency k, (seqB, kB2) frequency N-k.
-    auto unit = [&](int seqA, int kA2, int seqB, int kB2) {
-        const int oA = seqA * SEQS + padi(kA2), oB = seqB * SEQS + padi(kB2);
-        const float2 zA = buf[oA], zB = buf[oB];
-        const float2 wA = cmul(wrow[seqA], tw14[kA2]);
-        const float2 wB = cmul(wrow[seqB], tw14[kB2]);
-        const float2 xA = r2c_split(zA, zB, wA);
-        const float2 xB = r2c_split(zB, zA, wB);
-        const size_t gA = (size_t)rows[seqA] * N2 + kA2, gB = (size_t)rows[seqB] * N2 + kB2;
-        if constexpr (MODE == ROWS_SEEDPREP) {
-            Sst[gA] = xA;
-            Sst[gB] = xB;
-        } else {
-            const float2 pA = cmul(xA, Sst[gA]);
-            const float2 pB = cmul(xB, Sst[gB]);
-            buf[oA] = conjf2(c2r_split(pA, pB, wA));
-            buf[oB] = conjf2(c2r_split(pB, pA, wB));
-        }
+        for (int r = 0; r < 32; ++r) v[r] = rp[256 * r];
+    };
+    // forward stages 1 (inputs in v) .. 3; after stage 1 has consumed v, refill(v) requests the
+    // next transform's inputs into the same registers; ends with this thread's stage-3 outputs
+    auto forward = [&](float2 (&v)[32], const RowMap& m, float2 (&zA)[16], float2 (&zB)[16],
+                       auto&& refill) {
+        stage_first_to_lds<32>(v, buf + s1seq * SEQS, s1j);
+        refill(v);
+        __syncthreads();
+        stockham_stage<16, 32, ROW_LOG, 2, SEQS, NT>(buf, twl);
+        stage_last_from_lds<16, N2>(zA, buf + m.sA * SEQS, m.jA, twl);
+        stage_last_from_lds<16, N2>(zB, buf + m.sB * SEQS, m.jB, twl);
     };
 
-#pragma unroll 1
-    for (int p = tid; p < N2; p += NT) {
-        if (w != 0) {
-            unit(0, p, 1, N2 - 1 - p);
-        } else if (p >= N2 / 2) {
-            const int q = p - N2 / 2;
-            unit(1, q, 1, N2 - 1 - q);
-        } else if (p != 0) {
-            unit(0, p, 0, N2 - p);
-        } else {
-            // k = 0: DC zeroed, Nyquist X[N] = Re Z0 - Im Z0 carried separately; and the
-            // self-paired k = N/2 at (row 0, N2/2).
-            const float2 z0 = buf[padi(0)];
-            const float2 xN = make_float2(z0.x - z0.y, 0.f);
-            const int oh = padi(N2 / 2);
-            const float2 zh = buf[oh];
-            const float2 wh = cmul(wrow[0], tw14[N2 / 2]);
-            const float2 xh = r2c_split(zh, zh, wh);
-            const size_t gh = (size_t)(N2 / 2);
-            if constexpr (MODE == ROWS_SEEDPREP) {
-                Sst[0] = make_float2(0.f, 0.f);
-                Sst[(size_t)N1 * N2] = xN;
-                Sst[gh] = xh;
+    uint32_t w = blockIdx.x;
+    if (w >= items) return;
+    // vmcnt retires in issue order, so a prefetch only overlaps work if no later-issued global
+    // load is waited on before the prefetched data is used: the small gathers of the next item
+    // (row twiddles) are issued and consumed before its prefetch.
+    RowMap m = row_map(w, N1, tid);
+    float2 wrA = tw_two_level(twlo, twhi, m.rowA), wrB = tw_two_level(twlo, twhi, m.rowB);
+    float2 pre[32];  // prefetched stage-1 inputs of the next transform
+    s1_load(MODE == ROWS_STATIC ? Yk : Sst, w, pre);
+    __syncthreads();  // twl
+
+    while (true) {
+        auto wkA = [&](int r) { return cmul(wrA, tw_lds(twl, m.jA + 512 * r)); };
+        auto wkB = [&](int r) { return cmul(wrB, tw_lds(twl, m.jB + 512 * r)); };
+        const uint32_t wn = w + gridDim.x;
+        const bool more = wn < items;
+        RowMap mn = m;
+        float2 wrAn = wrA, wrBn = wrB;
+
+        float2 SA[16], SB[16];
+        float2 Sn = make_float2(0.f, 0.f);
+        if constexpr (MODE == ROWS_SEEDPREP || MODE == ROWS_DYN) {
+            float2 zA[16], zB[16];
+            // next request: the key rows of this item (DYN) or the seed rows of the next item
+            forward(pre, m, zA, zB, [&](float2 (&v)[32]) {
+                if (MODE == ROWS_DYN) s1_load(Yk, w, v);
+                else if (more) s1_load(Sst, wn, v);
+            });
+            if (!m.special) {
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    SA[r] = r2c_split(zA[r], zB[15 - r], wkA(r));
+                    SB[15 - r] = r2c_split(zB[15 - r], zA[r], wkB(15 - r));
+                }
             } else {
-                const float2 pN = cmul(xN, Sst[(size_t)N1 * N2]);
-                buf[padi(0)] = conjf2(c2r_split(make_float2(0.f, 0.f), pN, make_float2(1.f, 0.f)));
-                const float2 ph = cmul(xh, Sst[gh]);
-                buf[oh] = conjf2(c2r_split(ph, ph, wh));
+                Sn = make_float2(zA[0].x - zA[0].y, 0.f);
+                SA[0] = make_float2(0.f, 0.f);
+                SA[8] = r2c_split(zA[8], zA[8], wkA(8));
+#pragma unroll
+                for (int r = 1; r < 8; ++r) {
+                    SA[r] = r2c_split(zA[r], zA[16 - r], wkA(r));
+                    SA[16 - r] = r2c_split(zA[16 - r], zA[r], wkA(16 - r));
+                }
+#pragma unroll
+                for (int r = 0; r < 16; ++r) SB[r] = r2c_split(zB[r], zB[15 - r], wkB(r));
+            }
+            __syncthreads();  // stage-3 reads done before LDS is reused
+            if constexpr (MODE == ROWS_SEEDPREP) {
+                // in place: every thread of the workgroup has consumed these rows
+                if (more) {
+                    mn = row_map(wn, N1, tid);
+                    wrAn = tw_two_level(twlo, twhi, mn.rowA);
+                    wrBn = tw_two_level(twlo, twhi, mn.rowB);
+                }
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    Sst[(size_t)m.rowA * N2 + m.jA + 512 * r] = SA[r];
+                    Sst[(size_t)m.rowB * N2 + m.jB + 512 * r] = SB[r];
+                }
+                if (m.special) Sst[(size_t)N1 * N2] = Sn;
+                if (!more) break;
+                w = wn;
+                m = mn;
+                wrA = wrAn;
+                wrB = wrBn;
+                continue;
             }
         }
-    }
-    if constexpr (MODE == ROWS_SEEDPREP) return;
-    __syncthreads();
-    lds_fft<LG2, 2, SEQS, NT>(buf, tw14);
-#pragma unroll 2
-    for (int i = 0; i < PER; ++i) {
-        const int seq = i / HALF, t2 = tid + (i % HALF) * NT;
-        const uint32_t row = rows[seq];
-        const float2 F = buf[seq * SEQS + padi(t2)];
-        Yk[(size_t)row * N2 + t2] = cmul(F, tw_two_level(twlo, twhi, 2u * (uint32_t)t2 * row));
+
+        // key rows: forward, split, product, inverse split -> conj(Q) at the same positions
+        float2 qA[16], qB[16];
+        {
+            float2 zA[16], zB[16];
+            forward(pre, m, zA, zB, [&](float2 (&)[32]) {
+                if constexpr (MODE == ROWS_STATIC) {
+                    // the resident spectrum of this item is requested now, used after the FFT
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        SA[r] = Sst[(size_t)m.rowA * N2 + m.jA + 512 * r];
+                        SB[r] = Sst[(size_t)m.rowB * N2 + m.jB + 512 * r];
+                    }
+                    if (m.special) Sn = Sst[(size_t)N1 * N2];
+                }
+            });
+            auto unit = [&](float2 za, float2 zb, float2 sa, float2 sb, float2 wa, float2 wb, float2& qa,
+                            float2& qb) {
+                const float2 pa = cmul(r2c_split(za, zb, wa), sa);
+                const float2 pb = cmul(r2c_split(zb, za, wb), sb);
+                qa = conjf2(c2r_split(pa, pb, wa));
+                qb = conjf2(c2r_split(pb, pa, wb));
+            };
+            if (!m.special) {
+#pragma unroll
+                for (int r = 0; r < 16; ++r)
+                    unit(zA[r], zB[15 - r], SA[r], SB[15 - r], wkA(r), wkB(15 - r), qA[r], qB[15 - r]);
+            } else {
+                const float2 pn = cmul(make_float2(zA[0].x - zA[0].y, 0.f), Sn);
+                qA[0] = conjf2(c2r_split(make_float2(0.f, 0.f), pn, make_float2(1.f, 0.f)));
+                {
+                    const float2 w8 = wkA(8);
+                    const float2 p8 = cmul(r2c_split(zA[8], zA[8], w8), SA[8]);
+                    qA[8] = conjf2(c2r_split(p8, p8, w8));
+                }
+#pragma unroll
+                for (int r = 1; r < 8; ++r)
+                    unit(zA[r], zA[16 - r], SA[r], SA[16 - r], wkA(r), wkA(16 - r), qA[r], qA[16 - r]);
+#pragma unroll
+                for (int r = 0; r < 8; ++r)
+                    unit(zB[r], zB[15 - r], SB[r], SB[15 - r], wkB(r), wkB(15 - r), qB[r], qB[15 - r]);
+            }
+        }
+        __syncthreads();  // all forward stage-3 reads are done before LDS is overwritten
+        stage_first_to_lds<16>(qA, buf + m.sA * SEQS, m.jA);
+        stage_first_to_lds<16>(qB, buf + m.sB * SEQS, m.jB);
+        __syncthreads();
+        stockham_stage<16, 16, ROW_LOG, 2, SEQS, NT>(buf, twl);
+        {
+            float2 F[32];
+            stage_last_from_lds<32, N2>(F, buf + s1seq * SEQS, s1j, twl);
+            // R'[row][t2] = F[t2] w_N^{t2 row}, t2 = j + 256 r:  w_L^{2 row j} * w_L^{512 row r}
+            const uint32_t row = s1seq ? m.row1 : m.row0;
+            const float2 base = tw_two_level(twlo, twhi, (2u * row * (uint32_t)s1j) & (g.L - 1u));
+            const float2 step = tw_two_level(twlo, twhi, (512u * row) & (g.L - 1u));
+            float2 anchor[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+                anchor[a] = tw_two_level(twlo, twhi, (512u * row * (uint32_t)(8 * (a + 1))) & (g.L - 1u));
+            if (more) {
+                mn = row_map(wn, N1, tid);
+                wrAn = tw_two_level(twlo, twhi, mn.rowA);
+                wrBn = tw_two_level(twlo, twhi, mn.rowB);
+            }
+            float2* op = Yk + (size_t)row * N2 + s1j;
+            float2 tw = base;
+#pragma unroll
+            for (int r = 0; r < 32; ++r) {
+                if ((r & 7) == 0 && r)  // re-anchor every 8 steps to bound the recurrence error
+                    tw = cmul(base, anchor[r / 8 - 1]);
+                op[256 * r] = cmul(F[r], tw);
+                tw = cmul(tw, step);
+            }
+        }
+        __syncthreads();  // stage-3 reads done before the next item writes LDS
+        if (!more) break;
+        // next request: the first operand of the next item (issued here rather than during the
+        // inverse transform, whose registers are full)
+        s1_load(MODE == ROWS_STATIC ? Yk : Sst, wn, pre);
+        w = wn;
+        m = mn;
+        wrA = wrAn;
+        wrB = wrBn;
     }
 }
 
 // ------------------------------------------------------------------ pass A' ----
 // Workgroup grp owns columns [W*grp, W*grp + W): G[t1] = FFT_N1(R'[.][t2])[t1] = conj of the
 // unnormalised inverse, so c[2t] = Re G / N and c[2t+1] = -Im G / N at t = t2 + N2*t1.
-// Writes one nibble (4 conv bits, MSB first) per (t1, column pair) inside the output window.
+// Writes one nibble (4 conv bits, MSB first) per (column pair, t1) inside the output window, in
+// column-pair-major order nibT[pair * N1 + t1] so that each workgroup's writes are contiguous.
 template <int LG1>
 __global__ __launch_bounds__(inv_cols_threads(LG1)) void k_inv_cols(
-    const float2* __restrict__ Rp, uint8_t* __restrict__ nib, Geom g,
-    const uint32_t* __restrict__ wk_ptr, const uint32_t* __restrict__ ws_ptr,
+    const float2* __restrict__ Rp, uint8_t* __restrict__ nibT, Geom g,
+    const uint32_t* __restrict__ wk_cnt, const uint32_t* __restrict__ ws_cnt,
     const float2* __restrict__ tw14, float* __restrict__ margin) {
     constexpr int N1 = 1 << LG1;
     constexpr int W = inv_cols_width(LG1);
     constexpr int NT = inv_cols_threads(LG1);
     constexpr int SEQS = seq_stride(N1);
     __shared__ float2 buf[W * SEQS];
+    __shared__ float2 twl[TWL_SIZE];
+    load_twl(twl, tw14, NT);
     const uint32_t grp = xcd_logical(blockIdx.x, gridDim.x);
-    for (int e = threadIdx.x; e < W * N1; e += NT) {
-        const int col = e % W, k1 = e / W;
-        buf[col * SEQS + padi(k1)] = Rp[(size_t)k1 * g.N2 + grp * W + col];
+    if constexpr (LG1 == 13) {
+        // plan 32 x 16 x 16 over the two columns: stage 1 straight from global memory (thread t:
+        // column t & 1, butterfly t >> 1, rows j + 256 r), stage 3 into registers (thread t:
+        // butterfly t of both columns, t1 = t + 512 r), then the epilogue.
+        static_assert(W == 2 && NT == 512, "fused column pass expects 2 columns, 512 threads");
+        {
+            const int col = threadIdx.x & 1, j = threadIdx.x >> 1;
+            const float2* rp = Rp + (size_t)j * g.N2 + grp * W + col;
+            float2 v[32];
+#pragma unroll
+            for (int r = 0; r < 32; ++r) v[r] = rp[(size_t)256 * r * g.N2];
+            __syncthreads();  // twl
+            stage_first_to_lds<32>(v, buf + col * SEQS, j);
+        }
+        __syncthreads();
+        stockham_stage<16, 32, LG1, 2, SEQS, NT>(buf, twl);
+        float2 G0[16], G1[16];
+        stage_last_from_lds<16, N1>(G0, buf, threadIdx.x, twl);
+        stage_last_from_lds<16, N1>(G1, buf + SEQS, threadIdx.x, twl);
+        const float corr = correction(pop_total(wk_cnt), pop_total(ws_cnt), g.L);
+        const float invN = g.inv_N;
+        float dist = 0.f;
+        const uint32_t t2 = grp * W;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const uint32_t t1 = threadIdx.x + 512u * r;
+            const uint32_t i0 = 2u * (t2 + g.N2 * t1);
+            if (i0 + 4u <= g.out_off || i0 >= g.out_off + g.out_bits) continue;
+            const uint32_t b0 = round_bit(G0[r].x * invN + corr, dist);
+            const uint32_t b1 = round_bit(-G0[r].y * invN + corr, dist);
+            const uint32_t b2 = round_bit(G1[r].x * invN + corr, dist);
+            const uint32_t b3 = round_bit(-G1[r].y * invN + corr, dist);
+            nibT[(size_t)grp * N1 + t1] = (uint8_t)((b0 << 3) | (b1 << 2) | (b2 << 1) | b3);
+        }
+        if (margin) margin_publish(dist, margin);
+        return;
+    }
+    constexpr int LPT = (W * N1 + NT - 1) / NT;
+    float2 v[LPT];
+#pragma unroll
+    for (int i = 0; i < LPT; ++i) {
+        const int e = threadIdx.x + i * NT;
+        if (e < W * N1) v[i] = Rp[(size_t)(e / W) * g.N2 + grp * W + (e % W)];
+    }
+#pragma unroll
+    for (int i = 0; i < LPT; ++i) {
+        const int e = threadIdx.x + i * NT;
+        if (e < W * N1) buf[(e % W) * SEQS + padi(e / W)] = v[i];
     }
     __syncthreads();
-    lds_fft<LG1, W, SEQS, NT>(buf, tw14);
-    const float corr = correction(*wk_ptr, *ws_ptr, g.L);
+    lds_fft<LG1, W, SEQS, NT>(buf, twl);
+    const float corr = correction(pop_total(wk_cnt), pop_total(ws_cnt), g.L);
     const float invN = g.inv_N;
     float dist = 0.f;
     constexpr int P = W / 2;
+    // the output window [out_off, out_off + out_bits) as a t1 range for this pair of columns
     for (int e = threadIdx.x; e < P * N1; e += NT) {
-        const int p = e % P, t1 = e / P;
+        const int p = e / N1, t1 = e % N1;
         const uint32_t t2 = grp * W + 2u * (uint32_t)p;
         const uint32_t i0 = 2u * (t2 + g.N2 * (uint32_t)t1);
         if (i0 + 4u <= g.out_off || i0 >= g.out_off + g.out_bits) continue;
@@ -276,29 +534,47 @@ __global__ __launch_bounds__(inv_cols_threads(LG1)) void k_inv_cols(
         const uint32_t b1 = round_bit(-G0.y * invN + corr, dist);
         const uint32_t b2 = round_bit(G1.x * invN + corr, dist);
         const uint32_t b3 = round_bit(-G1.y * invN + corr, dist);
-        nib[i0 >> 2] = (uint8_t)((b0 << 3) | (b1 << 2) | (b2 << 1) | b3);
+        nibT[(size_t)(grp * P + (uint32_t)p) * N1 + (uint32_t)t1] = (uint8_t)((b0 << 3) | (b1 << 2) | (b2 << 1) | b3);
     }
     if (margin) margin_publish(dist, margin);
 }
 
 // ------------------------------------------------------------------ output ----
-// Output word w: conv bits [out_off + 32w, +32) from 8 nibbles, XOR key_rest word w
-// ((key[h/32+w] << 1) | (key[h/32+w+1] >> 31), key2StartRest PrivacyAmplification.cu:873-878),
-// stored byte-swapped for the OutBytes layout (ToBinaryArray, PrivacyAmplification.cu:757-762).
-__global__ __launch_bounds__(256) void k_pack(const uint8_t* __restrict__ nib,
+// The conv bit stream in nibbles: nibble q (bits 4q..4q+3) is nibT[(q mod P2) * N1 + q / P2] with
+// P2 = N2/2 column pairs.  A workgroup transposes a tile of PACK_T1 rows x PACK_G pairs through
+// LDS and writes the output words whose 8 nibbles lie in the tile: conv bits
+// [out_off + 32w, +32), XOR key_rest word w ((key[h/32+w] << 1) | (key[h/32+w+1] >> 31),
+// key2StartRest PrivacyAmplification.cu:873-878), byte-swapped for the OutBytes layout
+// (ToBinaryArray, PrivacyAmplification.cu:757-762).
+__global__ __launch_bounds__(256) void k_pack(const uint8_t* __restrict__ nibT,
                                               const uint32_t* __restrict__ key,
                                               uint32_t* __restrict__ out, Geom g) {
-    const uint32_t w = blockIdx.x * 256u + threadIdx.x;
-    if (w >= (g.out_bits >> 5)) return;
-    const uint32_t q = (g.out_off >> 2) + 8u * w;
-    const uint2 nb = *reinterpret_cast<const uint2*>(nib + q);
-    uint32_t cw = 0;
+    __shared__ uint8_t tile[PACK_T1][PACK_G + 4];
+    const uint32_t P2 = g.N2 >> 1;
+    const uint32_t q_lo = g.out_off >> 2, q_hi = (g.out_off + g.out_bits) >> 2;
+    const uint32_t row0 = ((q_lo / P2) & ~3u) + blockIdx.y * PACK_T1;  // 4-aligned for 32-bit reads
+    const uint32_t g0 = blockIdx.x * PACK_G;
+    // load: PACK_G pairs x PACK_T1 rows, 4 rows per 32-bit read
+    for (uint32_t e = threadIdx.x; e < PACK_G * (PACK_T1 / 4); e += 256) {
+        const uint32_t gp = e / (PACK_T1 / 4), r4 = (e % (PACK_T1 / 4)) * 4;
+        const uint32_t row = row0 + r4;
+        uint32_t v = 0;
+        if (row < g.N1) v = *reinterpret_cast<const uint32_t*>(nibT + (size_t)(g0 + gp) * g.N1 + row);
 #pragma unroll
-    for (int j = 0; j < 4; ++j) cw |= ((nb.x >> (8 * j)) & 0xFu) << (28 - 4 * j);
+        for (int k = 0; k < 4; ++k) tile[r4 + k][gp] = (uint8_t)(v >> (8 * k));
+    }
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < PACK_T1 * (PACK_G / 8); e += 256) {
+        const uint32_t r = e / (PACK_G / 8), gw = (e % (PACK_G / 8)) * 8;
+        const uint32_t q = (row0 + r) * P2 + g0 + gw;
+        if (q < q_lo || q >= q_hi) continue;
+        const uint32_t w = (q - q_lo) >> 3;
+        uint32_t cw = 0;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) cw |= ((nb.y >> (8 * j)) & 0xFu) << (12 - 4 * j);
-    if (g.do_xor && 32u * w < g.m) cw ^= (key[g.key_hw + w] << 1) | (key[g.key_hw + w + 1] >> 31);
-    out[w] = g.reverse_endian ? __builtin_bswap32(cw) : cw;
+        for (int j = 0; j < 8; ++j) cw |= ((uint32_t)tile[r][gw + j] & 0xFu) << (28 - 4 * j);
+        if (g.do_xor && 32u * w < g.m) cw ^= (key[g.key_hw + w] << 1) | (key[g.key_hw + w + 1] >> 31);
+        out[w] = g.reverse_endian ? __builtin_bswap32(cw) : cw;
+    }
 }
 
 // ------------------------------------------------------------------ small path ----
@@ -316,8 +592,10 @@ __global__ __launch_bounds__(small_threads(LGN)) void k_small(
     constexpr int SEQS = seq_stride(N);
     constexpr int PER = (N + NT - 1) / NT;
     __shared__ float2 buf[2 * SEQS];
+    __shared__ float2 twl[TWL_SIZE];
     __shared__ uint32_t cnt[2];
     const int tid = threadIdx.x;
+    load_twl(twl, tw14, NT);
     const bool dyn = SEEDPREP || (sspec == nullptr);
     if (tid < 2) cnt[tid] = 0u;
     __syncthreads();
@@ -343,11 +621,11 @@ __global__ __launch_bounds__(small_threads(LGN)) void k_small(
     if (cs) atomicAdd(&cnt[1], cs);
     __syncthreads();
     if (SEEDPREP) {
-        lds_fft<LGN, 2, SEQS, NT>(buf, tw14);  // sequence 0 is unused
+        lds_fft<LGN, 2, SEQS, NT>(buf, twl);  // sequence 0 is unused
     } else if (dyn) {
-        lds_fft<LGN, 2, SEQS, NT>(buf, tw14);
+        lds_fft<LGN, 2, SEQS, NT>(buf, twl);
     } else {
-        lds_fft<LGN, 1, SEQS, NT>(buf, tw14);
+        lds_fft<LGN, 1, SEQS, NT>(buf, twl);
     }
     constexpr int TWS = TW14 / L;
     float2 X[PER];
@@ -357,7 +635,7 @@ __global__ __launch_bounds__(small_threads(LGN)) void k_small(
         const int k = tid + i * NT;
         if (k >= N) continue;
         const int kp = (N - k) & (N - 1);
-        const float2 wk = tw14[k * TWS];
+        const float2 wk = tw_lds(twl, k * TWS);
         float2 s, sn = make_float2(0.f, 0.f);
         if (dyn) {
             const float2 zk = buf[SEQS + padi(k)], zp = buf[SEQS + padi(kp)];
@@ -384,7 +662,7 @@ __global__ __launch_bounds__(small_threads(LGN)) void k_small(
         X[i] = cmul(x, s);
     }
     if (SEEDPREP) {
-        if (tid == 0) *ws_out = cnt[1];
+        if (tid < POP_SLOTS) ws_out[tid] = (tid == 0) ? cnt[1] : 0u;
         return;
     }
     __syncthreads();
@@ -400,7 +678,7 @@ __global__ __launch_bounds__(small_threads(LGN)) void k_small(
         if (k >= N) continue;
         const int kp = (N - k) & (N - 1);
         const float2 pp = (k == 0) ? PN : buf[padi(kp)];
-        X[i] = conjf2(c2r_split(X[i], pp, tw14[k * TWS]));
+        X[i] = conjf2(c2r_split(X[i], pp, tw_lds(twl, k * TWS)));
     }
     __syncthreads();
 #pragma unroll
@@ -409,8 +687,8 @@ __global__ __launch_bounds__(small_threads(LGN)) void k_small(
         if (k < N) buf[padi(k)] = X[i];
     }
     __syncthreads();
-    lds_fft<LGN, 1, SEQS, NT>(buf, tw14);
-    const uint32_t ws = dyn ? cnt[1] : *ws_ptr;
+    lds_fft<LGN, 1, SEQS, NT>(buf, twl);
+    const uint32_t ws = dyn ? cnt[1] : pop_total(ws_ptr);
     const float corr = correction(cnt[0], ws, (uint32_t)L);
     const float invN = g.inv_N;
     float dist = 0.f;
@@ -435,7 +713,7 @@ namespace {
 template <int LGM, int D>
 hipError_t launch_fwd_cols_t(const uint32_t* tb, float2* Y, const Geom& g, const Tables& t, hipStream_t s) {
     const uint32_t grid = (g.N2 / 16u) * (uint32_t)D;
-    k_fwd_cols<LGM, D><<<grid, 256, 0, s>>>(tb, Y, g, t.tw14, t.lo, t.hi);
+    k_fwd_cols<LGM, D><<<grid, FWD_NT, 0, s>>>(tb, Y, g, t.tw14, t.lo, t.hi);
     return hipGetLastError();
 }
 template <int LG1>
@@ -449,7 +727,8 @@ template <int D>
 hipError_t launch_transpose_t(const uint32_t* bits, uint32_t* tb, uint32_t* count, const Geom& g,
                               int is_seed, hipStream_t s) {
     const uint32_t total = g.M * g.N2;
-    k_transpose_bits<D><<<(total + 255u) / 256u, 256, 0, s>>>(bits, tb, count, g, is_seed);
+    const uint32_t per_block = 256u * TB_PER;
+    k_transpose_bits<D><<<(total + per_block - 1u) / per_block, 256, 0, s>>>(bits, tb, count, g, is_seed);
     return hipGetLastError();
 }
 template <int LGN>
@@ -467,8 +746,7 @@ hipError_t launch_small_t(const uint32_t* key, const uint32_t* seed, const float
 }  // namespace
 
 hipError_t launch_zero_u32(uint32_t* p, uint32_t count, hipStream_t s) {
-    if (count > 64) return hipErrorInvalidValue;
-    k_zero_u32<<<1, 64, 0, s>>>(p, count);
+    k_zero_u32<<<1, 256, 0, s>>>(p, count);
     return hipGetLastError();
 }
 
@@ -503,10 +781,12 @@ hipError_t launch_fwd_cols(const uint32_t* tb, float2* Y, const Geom& g, const T
     }
 }
 
-hipError_t launch_rows(int mode, float2* Yk, float2* S, const Geom& g, const Tables& t, hipStream_t s) {
-    const uint32_t grid = g.N1 / 2u;
+hipError_t launch_rows(int mode, float2* Yk, float2* S, const Geom& g, const Tables& t, int cus, hipStream_t s) {
+    const uint32_t items = g.N1 / 2u;
+    const uint32_t grid = items < (uint32_t)cus ? items : (uint32_t)cus;  // one workgroup per CU
     switch (mode) {
     case ROWS_STATIC: k_rows<ROWS_STATIC><<<grid, ROWS_NT, 0, s>>>(Yk, S, g, t.tw14, t.lo, t.hi); break;
+    case ROWS_DYN: k_rows<ROWS_DYN><<<grid, ROWS_NT, 0, s>>>(Yk, S, g, t.tw14, t.lo, t.hi); break;
     case ROWS_SEEDPREP: k_rows<ROWS_SEEDPREP><<<grid, ROWS_NT, 0, s>>>(Yk, S, g, t.tw14, t.lo, t.hi); break;
     default: return hipErrorInvalidValue;
     }
@@ -533,9 +813,12 @@ hipError_t launch_inv_cols(const float2* R, uint8_t* nib, const Geom& g, const u
     }
 }
 
-hipError_t launch_pack(const uint8_t* nib, const uint32_t* key, uint32_t* out, const Geom& g, hipStream_t s) {
-    const uint32_t words = g.out_bits >> 5;
-    k_pack<<<(words + 255u) / 256u, 256, 0, s>>>(nib, key, out, g);
+hipError_t launch_pack(const uint8_t* nibT, const uint32_t* key, uint32_t* out, const Geom& g, hipStream_t s) {
+    const uint32_t P2 = g.N2 >> 1;
+    const uint32_t q_lo = g.out_off >> 2, q_hi = (g.out_off + g.out_bits) >> 2;
+    const uint32_t rows = (q_hi - 1u) / P2 - ((q_lo / P2) & ~3u) + 1u;
+    dim3 grid(P2 / PACK_G, (rows + PACK_T1 - 1u) / PACK_T1);
+    k_pack<<<grid, 256, 0, s>>>(nibT, key, out, g);
     return hipGetLastError();
 }
 

@@ -7,8 +7,13 @@
 
 namespace pa {
 
-enum { ROWS_STATIC = 1, ROWS_SEEDPREP = 2 };
+enum { ROWS_DYN = 0, ROWS_STATIC = 1, ROWS_SEEDPREP = 2 };
 constexpr int ROWS_NT = 512;  // threads of the row pass (256-VGPR budget per lane)
+constexpr int FWD_NT = 256;   // threads of the forward column pass
+constexpr int TB_PER = 4;     // bit-transpose outputs per thread
+constexpr int POP_SLOTS = 64; // partial popcount counters per operand
+constexpr int PACK_T1 = 32;   // output packer tile: rows x column pairs
+constexpr int PACK_G = 256;
 
 // Device twiddle tables: tw14[x] = exp(-2 pi i x / 2^14); lo/hi: exp(-2 pi i x / L) split as
 // lo[x mod 2^14] * hi[x >> 14].
@@ -24,7 +29,7 @@ __host__ __device__ constexpr int inv_cols_width(int lg1) {
 }
 __host__ __device__ constexpr int clamp_threads(int t) { return t < 64 ? 64 : (t > 1024 ? 1024 : t); }
 __host__ __device__ constexpr int inv_cols_threads(int lg1) {
-    return clamp_threads(inv_cols_width(lg1) * (1 << lg1) / 16);
+    return lg1 == 13 ? 512 : clamp_threads(inv_cols_width(lg1) * (1 << lg1) / 16);
 }
 __host__ __device__ constexpr int small_threads(int lgn) { return clamp_threads(2 * (1 << lgn) / 16); }
 
@@ -34,7 +39,9 @@ hipError_t launch_transpose(const uint32_t* bits, uint32_t* tb, uint32_t* count,
 hipError_t launch_fwd_cols(const uint32_t* tb, float2* Y, const Geom& g, const Tables& t, hipStream_t s);
 // ROWS_SEEDPREP: S <- split spectrum of the seed rows held in S (in place, N+1 values).
 // ROWS_STATIC:   Yk <- R' from the key rows in Yk and the split seed spectrum S.
-hipError_t launch_rows(int mode, float2* Yk, float2* S, const Geom& g, const Tables& t, hipStream_t s);
+// ROWS_DYN:      Yk <- R' from the key rows in Yk and the seed rows (pass-A output) in S.
+// Persistent: at most `cus` workgroups (one per compute unit) walk the row pairs.
+hipError_t launch_rows(int mode, float2* Yk, float2* S, const Geom& g, const Tables& t, int cus, hipStream_t s);
 hipError_t launch_inv_cols(const float2* R, uint8_t* nib, const Geom& g, const uint32_t* wk,
                            const uint32_t* ws, const Tables& t, float* margin, hipStream_t s);
 hipError_t launch_pack(const uint8_t* nib, const uint32_t* key, uint32_t* out, const Geom& g, hipStream_t s);
