@@ -110,6 +110,37 @@ int pa_block_device_seeded(pa_engine* eng, const uint32_t* d_key, pa_seed_id see
                            uint32_t m, uint32_t flags, uint8_t* d_out, void* stream,
                            float* d_margin);
 
+/* One block of a batch.  For pa_batch_device every pointer is a device pointer; for pa_run_batch
+ * key, seed and out are host pointers and margin must be NULL. */
+typedef struct {
+    const uint32_t* key;   /* RefBits key: n/32+1 words (PA_XOR_KEY_REST) or h/32+1 words */
+    const uint32_t* seed;  /* n/32 seed words, or NULL to use the resident seed seed_id */
+    pa_seed_id seed_id;
+    uint32_t n, m, flags;
+    uint32_t reserved;     /* 0 */
+    uint8_t* out;          /* m/8 bytes (PA_COMPRESS) or n/16 bytes */
+    float* margin;         /* device float receiving max |v - rint(v)| (atomic max), or NULL */
+} pa_block;
+
+/* Batch options: PA_BATCH_GRAPH replays a captured hipGraph for a batch of the same shape
+ * (lengths classes, seed modes and counts); PA_BATCH_TUNE times the row-pass grid candidates of a
+ * shape not seen before and keeps the fastest. */
+enum pa_batch_flags {
+    PA_BATCH_GRAPH = 1u,
+    PA_BATCH_TUNE = 2u
+};
+
+/* Many blocks of uneven length in few launches: blocks are grouped by transform length and seed
+ * mode; each group runs as one launch per pass (grid z = block), and groups run concurrently on
+ * the engine's fork streams.  The batched form of the reference's bank of 2^27/n equal blocks per
+ * batched cuFFT plan (PrivacyAmplification_subblocks.cu:1058-1081, 1254-1403), here with any mix
+ * of n, m, flags and static/dynamic seeds.  Stream-ordered on `stream`, nothing synchronised.
+ * At most 65535 blocks per call. */
+int pa_batch_device(pa_engine* eng, const pa_block* blocks, uint32_t count, uint32_t batch_flags,
+                    void* stream);
+/* Same with host buffers; returns when every output has been written. */
+int pa_run_batch(pa_engine* eng, const pa_block* blocks, uint32_t count, uint32_t batch_flags);
+
 /* Per-stage device timing, the counterpart of the reference's STOPWATCH stage timers
  * (PrivacyAmplification.cu:146-166, 2053-2074).  When enabled, the engine records HIP events
  * around every kernel it launches (on the launch stream).  pa_stage_times synchronises, then

@@ -5,14 +5,18 @@ A drop-in for the Toeplitz-hashing stage of nicoboss/PrivacyAmplification: the s
 hand-written HIP kernels behind the C ABI declared in include/pa_engine.h.
 """
 from .engine import (  # noqa: F401
+    BATCH_GRAPH,
+    BATCH_TUNE,
     COMPRESS,
     DEFAULT_FLAGS,
     XOR_KEY_REST,
     Engine,
     EngineError,
+    PaBlock,
     build,
     library_path,
     load_library,
+    out_bytes,
     synth_bits,
     workspace_bytes,
 )

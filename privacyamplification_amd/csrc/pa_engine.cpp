@@ -14,7 +14,9 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <algorithm>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -96,11 +98,15 @@ int make_geom(uint32_t n, uint32_t m, uint32_t flags, uint32_t reverse_endian, u
 
 uint32_t key_words_for(const Geom& g) { return g.do_xor ? g.n / 32 + 1 : g.key_hw + 1; }
 
-// Per transform length: twiddles and the device scratch of one block.
-struct Workspace {
-    uint32_t L = 0;
+// Per transform length: w_L^x = lo[x mod 2^14] * hi[x >> 14].
+struct Twiddles {
     float2* lo = nullptr;      // 2^14 entries
     float2* hi = nullptr;      // max(1, L / 2^14) entries
+};
+
+// Per transform length: the device scratch of one (unbatched) block.
+struct Workspace {
+    uint32_t L = 0;
     float2* Yk = nullptr;      // N complex (key rows / R')
     float2* Ys = nullptr;      // N+1 complex (seed rows / split seed spectrum)
     uint32_t* tbk = nullptr;   // N/D words
@@ -137,6 +143,7 @@ struct pa_engine {
     hipStream_t stream = nullptr;
     float2* tw14 = nullptr;
     int cus = 256;  // compute units of the device (persistent kernels launch one workgroup each)
+    std::map<uint32_t, Twiddles> tw;
     std::map<uint32_t, std::unique_ptr<Workspace>> ws;
     std::map<pa_seed_id, Seed> seeds;
     std::map<pa_ticket, Ticket> tickets;
@@ -151,6 +158,18 @@ struct pa_engine {
     std::vector<hipEvent_t> event_pool;
     double stage_ms[16] = {};
     uint32_t stage_count[16] = {};
+    // batched launches
+    static constexpr int FORKS = 4;
+    hipStream_t fork[FORKS] = {};
+    hipEvent_t fork_ev = nullptr, join_ev[FORKS] = {};
+    uint8_t* arena = nullptr;                // grow-only device arena: table, scratch, staging
+    uint64_t arena_cap = 0;
+    pa::BlockArgs* htab = nullptr;           // pinned host copy of the block table
+    uint32_t htab_cap = 0;
+    hipEvent_t htab_free = nullptr;          // recorded after the last table upload
+    bool htab_pending = false;
+    std::map<std::string, hipGraphExec_t> graphs;
+    std::map<std::string, uint32_t> tuned;   // row-pass grid per (L, seed mode, count)
 };
 
 namespace {
@@ -193,21 +212,43 @@ uint64_t workspace_bytes(const Geom& g) {
     return b;
 }
 
+int get_twiddles(pa_engine* e, uint32_t L, Twiddles** out) {
+    auto it = e->tw.find(L);
+    if (it != e->tw.end()) {
+        *out = &it->second;
+        return PA_OK;
+    }
+    Twiddles t;
+    const uint64_t hi_n = L > 16384 ? L / 16384 : 1;
+    int rc = dev_alloc(e, (void**)&t.lo, 16384 * sizeof(float2));
+    if (rc != PA_OK) return rc;
+    rc = dev_alloc(e, (void**)&t.hi, hi_n * sizeof(float2));
+    if (rc != PA_OK) {
+        dev_free(e, t.lo, 16384 * sizeof(float2));
+        return rc;
+    }
+    auto lo = twiddles(L, 16384, 1);
+    auto hi = twiddles(L, hi_n, 16384);
+    PA_HIP(hipMemcpy(t.lo, lo.data(), lo.size() * sizeof(float2), hipMemcpyHostToDevice));
+    PA_HIP(hipMemcpy(t.hi, hi.data(), hi.size() * sizeof(float2), hipMemcpyHostToDevice));
+    *out = &e->tw.emplace(L, t).first->second;
+    return PA_OK;
+}
+
 int get_workspace(pa_engine* e, const Geom& g, Workspace** out) {
     auto it = e->ws.find(g.L);
     if (it != e->ws.end()) {
         *out = it->second.get();
         return PA_OK;
     }
+    Twiddles* tw;
+    int rc = get_twiddles(e, g.L, &tw);
+    if (rc != PA_OK) return rc;
     auto w = std::make_unique<Workspace>();
     w->L = g.L;
-    int rc;
-    const uint64_t hi_n = g.L > 16384 ? g.L / 16384 : 1;
     const size_t key_bytes = (size_t)(g.L / 32 + 2) * 4, out_bytes = (size_t)(g.L / 16 + 16);
     struct A { void** p; size_t b; };
     std::vector<A> allocs = {
-        {(void**)&w->lo, 16384 * sizeof(float2)},
-        {(void**)&w->hi, hi_n * sizeof(float2)},
         {(void**)&w->counts, 2 * pa::POP_SLOTS * sizeof(uint32_t)},
         {(void**)&w->d_key, key_bytes},
         {(void**)&w->d_seed, key_bytes},
@@ -229,16 +270,16 @@ int get_workspace(pa_engine* e, const Geom& g, Workspace** out) {
         }
         w->bytes += a.b;
     }
-    auto lo = twiddles(g.L, 16384, 1);
-    auto hi = twiddles(g.L, hi_n, 16384);
-    PA_HIP(hipMemcpy(w->lo, lo.data(), lo.size() * sizeof(float2), hipMemcpyHostToDevice));
-    PA_HIP(hipMemcpy(w->hi, hi.data(), hi.size() * sizeof(float2), hipMemcpyHostToDevice));
     *out = w.get();
     e->ws.emplace(g.L, std::move(w));
     return PA_OK;
 }
 
-pa::Tables tables(pa_engine* e, const Workspace* w) { return pa::Tables{e->tw14, w->lo, w->hi}; }
+// Twiddle tables of transform length L (get_twiddles must have created them).
+pa::Tables tables(pa_engine* e, uint32_t L) {
+    const Twiddles& t = e->tw.at(L);
+    return pa::Tables{e->tw14, t.lo, t.hi};
+}
 
 hipStream_t stream_of(pa_engine* e, void* s) { return s ? (hipStream_t)s : e->stream; }
 
@@ -277,7 +318,7 @@ hipError_t timed(pa_engine* e, int stage, hipStream_t s, F&& launch) {
 // The hot path on device buffers.  sspec/ws_dev != null selects the resident seed.
 int run_device(pa_engine* e, const Geom& g, Workspace* w, const uint32_t* key, const uint32_t* seed,
                const float2* sspec, const uint32_t* ws_dev, uint32_t* out, hipStream_t s, float* margin) {
-    const pa::Tables t = tables(e, w);
+    const pa::Tables t = tables(e, g.L);
     if (g.N1 == 1) {
         PA_HIP(timed(e, ST_SMALL, s, [&] {
             return pa::launch_small(key, seed, sspec, ws_dev, out, g, t, margin, nullptr, nullptr, s);
@@ -335,6 +376,12 @@ int pa_create(const pa_config* cfg, pa_engine** out) {
     PA_HIP(hipSetDevice(e->cfg.device));
     PA_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
     PA_HIP(hipDeviceGetAttribute(&e->cus, hipDeviceAttributeMultiprocessorCount, e->cfg.device));
+    for (int i = 0; i < pa_engine::FORKS; ++i) {
+        PA_HIP(hipStreamCreateWithFlags(&e->fork[i], hipStreamNonBlocking));
+        PA_HIP(hipEventCreateWithFlags(&e->join_ev[i], hipEventDisableTiming));
+    }
+    PA_HIP(hipEventCreateWithFlags(&e->fork_ev, hipEventDisableTiming));
+    PA_HIP(hipEventCreateWithFlags(&e->htab_free, hipEventDisableTiming));
     int rc = dev_alloc(e.get(), (void**)&e->tw14, 16384 * sizeof(float2));
     if (rc != PA_OK) return rc;
     auto tw = twiddles(16384, 16384, 1);
@@ -360,14 +407,28 @@ int pa_destroy(pa_engine* e) {
     }
     for (auto& kv : e->ws) {
         Workspace* w = kv.second.get();
-        void* ps[] = {w->lo, w->hi, w->Yk, w->Ys, w->tbk, w->tbs, w->nib, w->counts, w->d_key, w->d_seed, w->d_out};
+        void* ps[] = {w->Yk, w->Ys, w->tbk, w->tbs, w->nib, w->counts, w->d_key, w->d_seed, w->d_out};
         for (void* p : ps) (void)hipFree(p);
+    }
+    for (auto& kv : e->tw) {
+        (void)hipFree(kv.second.lo);
+        (void)hipFree(kv.second.hi);
     }
     for (auto& p : e->pending) {
         (void)hipEventDestroy(p.a);
         (void)hipEventDestroy(p.b);
     }
     for (hipEvent_t ev : e->event_pool) (void)hipEventDestroy(ev);
+    for (auto& kv : e->graphs) (void)hipGraphExecDestroy(kv.second);
+    for (int i = 0; i < pa_engine::FORKS; ++i) {
+        (void)hipStreamSynchronize(e->fork[i]);
+        (void)hipStreamDestroy(e->fork[i]);
+        (void)hipEventDestroy(e->join_ev[i]);
+    }
+    (void)hipEventDestroy(e->fork_ev);
+    (void)hipEventDestroy(e->htab_free);
+    (void)hipFree(e->arena);
+    (void)hipHostFree(e->htab);
     (void)hipFree(e->tw14);
     (void)hipStreamDestroy(e->stream);
     delete e;
@@ -400,7 +461,7 @@ int pa_seed_put(pa_engine* e, const uint32_t* seed, uint32_t n, uint32_t m, int3
     }
     hipStream_t s = e->stream;
     PA_HIP(hipMemcpyAsync(w->d_seed, seed, (size_t)(n / 32) * 4, hipMemcpyHostToDevice, s));
-    const pa::Tables t = tables(e, w);
+    const pa::Tables t = tables(e, g.L);
     if (g.N1 == 1) {
         PA_HIP(pa::launch_small(nullptr, w->d_seed, nullptr, nullptr, nullptr, g, t, nullptr, sd.spec, sd.ws, s));
     } else {
@@ -566,6 +627,342 @@ int pa_wait(pa_engine* e, pa_ticket ticket, int64_t timeout_ns) {
     (void)hipFree(t.dev_out);
     (void)hipFree(t.dev_key);
     (void)hipFree(t.dev_seed);
+    return PA_OK;
+}
+
+// ------------------------------------------------------------------ batched launches ----
+namespace {
+
+uint64_t align256(uint64_t x) { return (x + 255u) & ~uint64_t(255); }
+
+struct BatchItem {
+    Geom g{};
+    const Seed* sd = nullptr;  // resident seed (static mode)
+    bool dyn = true;
+};
+
+struct BatchGroup {
+    uint32_t L = 0;
+    bool dyn = true, small = false;
+    std::vector<uint32_t> idx;  // blocks of the group (caller order)
+    uint32_t tab0 = 0;          // first table row of the group
+    uint32_t tiles_y = 1;       // pack grid.y: the largest window of the group
+    uint32_t grid_x = 0;        // row-pass workgroups per block
+    double cost = 0.0;
+};
+
+struct BatchPlan {
+    std::vector<BatchItem> items;
+    std::vector<BatchGroup> groups;
+    uint64_t table_bytes = 0, scratch_bytes = 0;
+};
+
+// Device scratch of one block of a batch (the four-step path's intermediate buffers).
+uint64_t batch_scratch_bytes(const Geom& g, bool dyn) {
+    if (g.N1 == 1) return 0;
+    uint64_t b = align256((uint64_t)g.N * 8) + align256((uint64_t)(g.N / g.D) * 4);  // Yk, tbk
+    if (dyn) b += align256((uint64_t)(g.N + 1) * 8) + align256((uint64_t)(g.N / g.D) * 4);  // Ys, tbs
+    b += align256((uint64_t)g.L / 4 + 64) + align256(2 * pa::POP_SLOTS * 4);  // nib, counts
+    return b;
+}
+
+int batch_plan(pa_engine* e, const pa_block* blocks, uint32_t count, BatchPlan* p) {
+    if (!blocks || count == 0 || count > 65535u) return PA_ERR_ARG;
+    p->items.resize(count);
+    std::map<std::pair<uint32_t, bool>, uint32_t> gid;
+    for (uint32_t i = 0; i < count; ++i) {
+        const pa_block& b = blocks[i];
+        if (!b.key || !b.out || (!b.seed && !b.seed_id) || b.reserved) return PA_ERR_ARG;
+        BatchItem& it = p->items[i];
+        int rc = make_geom(b.n, b.m, b.flags, e->cfg.out_reverse_endian, e->cfg.max_n_log2, &it.g);
+        if (rc != PA_OK) return rc;
+        it.dyn = b.seed != nullptr;
+        if (!it.dyn) {
+            rc = seed_lookup(e, b.seed_id, it.g, &it.sd);
+            if (rc != PA_OK) return rc;
+        }
+        Twiddles* tw;
+        rc = get_twiddles(e, it.g.L, &tw);
+        if (rc != PA_OK) return rc;
+        const auto key = std::make_pair(it.g.L, it.dyn);
+        auto f = gid.find(key);
+        if (f == gid.end()) {
+            f = gid.emplace(key, (uint32_t)p->groups.size()).first;
+            BatchGroup gr;
+            gr.L = it.g.L;
+            gr.dyn = it.dyn;
+            gr.small = it.g.N1 == 1;
+            p->groups.push_back(gr);
+        }
+        BatchGroup& gr = p->groups[f->second];
+        gr.idx.push_back(i);
+        gr.cost += (double)it.g.L * (it.dyn ? 1.5 : 1.0);
+        if (!gr.small) gr.tiles_y = std::max(gr.tiles_y, pa::pack_tiles(it.g));
+        p->scratch_bytes += batch_scratch_bytes(it.g, it.dyn);
+    }
+    uint32_t row = 0;
+    for (auto& gr : p->groups) {
+        gr.tab0 = row;
+        row += (uint32_t)gr.idx.size();
+    }
+    p->table_bytes = align256((uint64_t)count * sizeof(pa::BlockArgs));
+    return PA_OK;
+}
+
+// Grows the batch arena to at least `bytes` (drains the device first; cached graphs refer to the
+// old arena and are dropped).
+int ensure_arena(pa_engine* e, uint64_t bytes) {
+    if (bytes <= e->arena_cap) return PA_OK;
+    PA_HIP(hipDeviceSynchronize());
+    for (auto& kv : e->graphs) (void)hipGraphExecDestroy(kv.second);
+    e->graphs.clear();
+    if (e->arena) dev_free(e, e->arena, e->arena_cap);
+    e->arena = nullptr;
+    e->arena_cap = 0;
+    const uint64_t cap = align256(bytes + bytes / 8);
+    int rc = dev_alloc(e, (void**)&e->arena, cap);
+    if (rc != PA_OK) return rc;
+    e->arena_cap = cap;
+    return PA_OK;
+}
+
+// All launches of one group on stream s.
+int batch_group_launch(pa_engine* e, const BatchPlan& p, const BatchGroup& gr, const pa::BlockArgs* tab,
+                       uint32_t grid_x, hipStream_t s) {
+    const pa::BlockArgs* bt = tab + gr.tab0;
+    const uint32_t cnt = (uint32_t)gr.idx.size();
+    const Geom& g0 = p.items[gr.idx[0]].g;
+    const pa::Tables t = tables(e, gr.L);
+    if (gr.small) {
+        PA_HIP(pa::launch_batch_small(bt, cnt, g0, t, s));
+        return PA_OK;
+    }
+    PA_HIP(pa::launch_batch_zero(bt, cnt, s));
+    PA_HIP(pa::launch_batch_transpose(bt, cnt, g0, 0, s));
+    PA_HIP(pa::launch_batch_fwd_cols(bt, cnt, g0, t, 0, s));
+    if (gr.dyn) {
+        PA_HIP(pa::launch_batch_transpose(bt, cnt, g0, 1, s));
+        PA_HIP(pa::launch_batch_fwd_cols(bt, cnt, g0, t, 1, s));
+    }
+    PA_HIP(pa::launch_batch_rows(gr.dyn ? pa::ROWS_DYN : pa::ROWS_STATIC, bt, cnt, g0, t, grid_x, s));
+    PA_HIP(pa::launch_batch_inv_cols(bt, cnt, g0, t, s));
+    PA_HIP(pa::launch_batch_pack(bt, cnt, g0, gr.tiles_y, s));
+    return PA_OK;
+}
+
+// Groups on the fork streams (largest first onto the least-loaded stream), joined back into s.
+int batch_launch_all(pa_engine* e, const BatchPlan& p, const pa::BlockArgs* tab, hipStream_t s) {
+    const size_t ng = p.groups.size();
+    if (ng == 1) return batch_group_launch(e, p, p.groups[0], tab, p.groups[0].grid_x, s);
+    std::vector<size_t> order(ng);
+    for (size_t i = 0; i < ng; ++i) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return p.groups[a].cost > p.groups[b].cost; });
+    double load[pa_engine::FORKS] = {};
+    bool used[pa_engine::FORKS] = {};
+    PA_HIP(hipEventRecord(e->fork_ev, s));
+    for (size_t gi : order) {
+        int k = 0;
+        for (int j = 1; j < pa_engine::FORKS; ++j)
+            if (load[j] < load[k]) k = j;
+        if (!used[k]) {
+            PA_HIP(hipStreamWaitEvent(e->fork[k], e->fork_ev, 0));
+            used[k] = true;
+        }
+        load[k] += p.groups[gi].cost;
+        int rc = batch_group_launch(e, p, p.groups[gi], tab, p.groups[gi].grid_x, e->fork[k]);
+        if (rc != PA_OK) return rc;
+    }
+    for (int k = 0; k < pa_engine::FORKS; ++k) {
+        if (!used[k]) continue;
+        PA_HIP(hipEventRecord(e->join_ev[k], e->fork[k]));
+        PA_HIP(hipStreamWaitEvent(s, e->join_ev[k], 0));
+    }
+    return PA_OK;
+}
+
+std::string tune_key(const BatchGroup& gr) {
+    return std::to_string(gr.L) + (gr.dyn ? "d" : "s") + std::to_string(gr.idx.size());
+}
+
+// Times the row-pass grid candidates of a group (whole group sequence per candidate; the outputs
+// are rewritten by the real run that follows) and records the fastest.
+int batch_tune_group(pa_engine* e, const BatchPlan& p, BatchGroup& gr, const pa::BlockArgs* tab, hipStream_t s) {
+    const uint32_t items = p.items[gr.idx[0]].g.N1 / 2u;
+    const uint32_t cnt = (uint32_t)gr.idx.size();
+    const uint32_t base = std::max<uint32_t>(1u, ((uint32_t)e->cus + cnt - 1u) / cnt);
+    hipEvent_t a, b;
+    PA_HIP(hipEventCreate(&a));
+    PA_HIP(hipEventCreate(&b));
+    float best = 1e30f;
+    uint32_t best_x = std::min(base, items);
+    for (uint32_t x = 1; x <= items; x *= 2) {
+        if (x * 4u < base || x > base * 8u) continue;
+        float ms = 0.f;
+        for (int rep = 0; rep < 2; ++rep) {  // first run warms up
+            PA_HIP(hipEventRecord(a, s));
+            int rc = batch_group_launch(e, p, gr, tab, x, s);
+            if (rc != PA_OK) return rc;
+            PA_HIP(hipEventRecord(b, s));
+            PA_HIP(hipEventSynchronize(b));
+            PA_HIP(hipEventElapsedTime(&ms, a, b));
+        }
+        if (ms < best) {
+            best = ms;
+            best_x = x;
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    e->tuned[tune_key(gr)] = best_x;
+    gr.grid_x = best_x;
+    return PA_OK;
+}
+
+// Fills and uploads the block table, then launches (tuning / graph replay as asked).
+// `dev` holds the device pointers of key, seed, out and margin of every block.
+int batch_execute(pa_engine* e, BatchPlan& p, const pa_block* dev, uint32_t bflags, hipStream_t s) {
+    const uint32_t count = (uint32_t)p.items.size();
+    pa::BlockArgs* dtab = reinterpret_cast<pa::BlockArgs*>(e->arena);
+    uint8_t* scratch = e->arena + p.table_bytes;
+    if (e->htab_pending) {
+        PA_HIP(hipEventSynchronize(e->htab_free));
+        e->htab_pending = false;
+    }
+    if (e->htab_cap < count) {
+        if (e->htab) (void)hipHostFree(e->htab);
+        e->htab = nullptr;
+        e->htab_cap = 0;
+        PA_HIP(hipHostMalloc((void**)&e->htab, (size_t)count * sizeof(pa::BlockArgs), hipHostMallocDefault));
+        e->htab_cap = count;
+    }
+    uint64_t off = 0;
+    for (auto& gr : p.groups) {
+        for (size_t j = 0; j < gr.idx.size(); ++j) {
+            const uint32_t i = gr.idx[j];
+            const BatchItem& it = p.items[i];
+            pa::BlockArgs a{};
+            a.g = it.g;
+            a.key = dev[i].key;
+            a.seed = it.dyn ? dev[i].seed : nullptr;
+            a.sspec = it.sd ? it.sd->spec : nullptr;
+            a.ws = it.sd ? it.sd->ws : nullptr;
+            a.out = reinterpret_cast<uint32_t*>(dev[i].out);
+            a.margin = dev[i].margin;
+            if (!gr.small) {
+                const Geom& g = it.g;
+                auto take = [&](uint64_t bytes) {
+                    uint8_t* q = scratch + off;
+                    off += align256(bytes);
+                    return q;
+                };
+                a.Yk = reinterpret_cast<float2*>(take((uint64_t)g.N * 8));
+                a.tbk = reinterpret_cast<uint32_t*>(take((uint64_t)(g.N / g.D) * 4));
+                if (it.dyn) {
+                    a.Ys = reinterpret_cast<float2*>(take((uint64_t)(g.N + 1) * 8));
+                    a.tbs = reinterpret_cast<uint32_t*>(take((uint64_t)(g.N / g.D) * 4));
+                }
+                a.nib = take((uint64_t)g.L / 4 + 64);
+                a.counts = reinterpret_cast<uint32_t*>(take(2 * pa::POP_SLOTS * 4));
+            }
+            e->htab[gr.tab0 + j] = a;
+        }
+        if (!gr.small) {
+            auto t = e->tuned.find(tune_key(gr));
+            const uint32_t items = p.items[gr.idx[0]].g.N1 / 2u;
+            const uint32_t cnt = (uint32_t)gr.idx.size();
+            gr.grid_x = t != e->tuned.end() ? t->second
+                                            : std::min(items, std::max<uint32_t>(1u, ((uint32_t)e->cus + cnt - 1u) / cnt));
+        }
+    }
+    PA_HIP(hipMemcpyAsync(dtab, e->htab, (size_t)count * sizeof(pa::BlockArgs), hipMemcpyHostToDevice, s));
+    PA_HIP(hipEventRecord(e->htab_free, s));
+    e->htab_pending = true;
+    if (bflags & PA_BATCH_TUNE)
+        for (auto& gr : p.groups)
+            if (!gr.small && e->tuned.find(tune_key(gr)) == e->tuned.end()) {
+                int rc = batch_tune_group(e, p, gr, dtab, s);
+                if (rc != PA_OK) return rc;
+            }
+    if (!(bflags & PA_BATCH_GRAPH)) return batch_launch_all(e, p, dtab, s);
+    std::string key = std::to_string((uintptr_t)e->arena) + ":";
+    for (auto& gr : p.groups)
+        key += tune_key(gr) + "x" + std::to_string(gr.grid_x) + "y" + std::to_string(gr.tiles_y) + ";";
+    auto f = e->graphs.find(key);
+    if (f == e->graphs.end()) {
+        hipGraph_t graph = nullptr;
+        PA_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        int rc = batch_launch_all(e, p, dtab, s);
+        hipError_t ec = hipStreamEndCapture(s, &graph);
+        if (rc != PA_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        PA_HIP(ec);
+        hipGraphExec_t exec = nullptr;
+        hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        PA_HIP(ei);
+        f = e->graphs.emplace(key, exec).first;
+    }
+    PA_HIP(hipGraphLaunch(f->second, s));
+    return PA_OK;
+}
+
+}  // namespace
+
+int pa_batch_device(pa_engine* e, const pa_block* blocks, uint32_t count, uint32_t batch_flags, void* stream) {
+    if (!e || (batch_flags & ~3u)) return PA_ERR_ARG;
+    std::lock_guard<std::mutex> lk(e->mu);
+    BatchPlan p;
+    int rc = batch_plan(e, blocks, count, &p);
+    if (rc != PA_OK) return rc;
+    rc = ensure_arena(e, p.table_bytes + p.scratch_bytes);
+    if (rc != PA_OK) return rc;
+    return batch_execute(e, p, blocks, batch_flags, stream_of(e, stream));
+}
+
+int pa_run_batch(pa_engine* e, const pa_block* blocks, uint32_t count, uint32_t batch_flags) {
+    if (!e || (batch_flags & ~3u)) return PA_ERR_ARG;
+    std::lock_guard<std::mutex> lk(e->mu);
+    BatchPlan p;
+    int rc = batch_plan(e, blocks, count, &p);
+    if (rc != PA_OK) return rc;
+    for (uint32_t i = 0; i < count; ++i)
+        if (blocks[i].margin) return PA_ERR_ARG;
+    // staging after the table and scratch: per block key (n/32+2 words), seed, output
+    std::vector<uint64_t> koff(count), soff(count), ooff(count);
+    uint64_t off = p.table_bytes + p.scratch_bytes, key_lo = off;
+    for (uint32_t i = 0; i < count; ++i) {
+        koff[i] = off;
+        off += align256((uint64_t)(p.items[i].g.n / 32 + 2) * 4);
+    }
+    const uint64_t key_hi = off;
+    for (uint32_t i = 0; i < count; ++i) {
+        soff[i] = off;
+        if (p.items[i].dyn) off += align256((uint64_t)(p.items[i].g.n / 32) * 4);
+        ooff[i] = off;
+        off += align256(out_bytes_of(p.items[i].g) + 16);
+    }
+    rc = ensure_arena(e, off);
+    if (rc != PA_OK) return rc;
+    hipStream_t s = e->stream;
+    std::vector<pa_block> dev(blocks, blocks + count);
+    PA_HIP(hipMemsetAsync(e->arena + key_lo, 0, key_hi - key_lo, s));
+    for (uint32_t i = 0; i < count; ++i) {
+        const Geom& g = p.items[i].g;
+        dev[i].key = reinterpret_cast<const uint32_t*>(e->arena + koff[i]);
+        PA_HIP(hipMemcpyAsync(e->arena + koff[i], blocks[i].key, (size_t)key_words_for(g) * 4, hipMemcpyHostToDevice, s));
+        if (p.items[i].dyn) {
+            dev[i].seed = reinterpret_cast<const uint32_t*>(e->arena + soff[i]);
+            PA_HIP(hipMemcpyAsync(e->arena + soff[i], blocks[i].seed, (size_t)(g.n / 32) * 4, hipMemcpyHostToDevice, s));
+        }
+        dev[i].out = e->arena + ooff[i];
+    }
+    rc = batch_execute(e, p, dev.data(), batch_flags, s);
+    if (rc != PA_OK) return rc;
+    for (uint32_t i = 0; i < count; ++i)
+        PA_HIP(hipMemcpyAsync(blocks[i].out, e->arena + ooff[i], out_bytes_of(p.items[i].g), hipMemcpyDeviceToHost, s));
+    PA_HIP(hipStreamSynchronize(s));
     return PA_OK;
 }
 

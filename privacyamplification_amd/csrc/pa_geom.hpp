@@ -14,6 +14,7 @@
 //   pass A' (inv_cols)  inverse column FFTs of length N1, W columns per workgroup, round-mod-2 epilogue.
 // N <= 8192 (L <= 2^14) runs fully on chip in one workgroup (small path).
 #pragma once
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace pa {
@@ -32,6 +33,24 @@ struct Geom {
     uint32_t key_hw;                   // h/32: index of the key word masked to its MSB
     uint32_t do_xor, reverse_endian;
     float inv_N;
+};
+
+// One block of a batched launch: its geometry and device buffers.  The blocks of one launch share
+// the transform length L (and so N1, N2, M, D, W); n, m, flags and the buffers are per block.
+struct BlockArgs {
+    Geom g;
+    const uint32_t* key;
+    const uint32_t* seed;   // dynamic seed, or nullptr for a resident one
+    const float2* sspec;    // resident split spectrum (N+1 values) when seed == nullptr
+    const uint32_t* ws;     // resident seed popcount partials when seed == nullptr
+    float2* Yk;             // N values (four-step path only)
+    float2* Ys;             // N+1 values (dynamic seed, four-step path only)
+    uint32_t* tbk;
+    uint32_t* tbs;
+    uint8_t* nib;
+    uint32_t* counts;       // key partial popcounts, then seed partial popcounts
+    uint32_t* out;
+    float* margin;          // may be nullptr
 };
 
 }  // namespace pa

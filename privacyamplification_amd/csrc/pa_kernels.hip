@@ -64,7 +64,10 @@ __device__ __forceinline__ uint32_t round_bit(float v, float& dist) {
     return (uint32_t)__float2int_rn(v) & 1u;
 }
 
-__global__ __launch_bounds__(256) void k_zero_u32(uint32_t* p, uint32_t count) {
+// Batched launches (BATCH = true) take each block's geometry and buffers from bt[blockIdx.z].
+template <bool BATCH>
+__global__ __launch_bounds__(256) void k_zero_u32(uint32_t* p, uint32_t count, const BlockArgs* __restrict__ bt) {
+    if constexpr (BATCH) p = bt[blockIdx.z].counts;
     for (uint32_t i = threadIdx.x; i < count; i += 256) p[i] = 0u;
 }
 
@@ -72,12 +75,20 @@ __global__ __launch_bounds__(256) void k_zero_u32(uint32_t* p, uint32_t count) {
 // tb[u*N2 + t2] = E | O << 16 with E bit v = x[2(t2 + N2(u + M v))], O bit v = the odd bit.
 // Each thread writes TB_PER outputs; popcounts are reduced per workgroup into one of POP_SLOTS
 // counters.
-template <int D>
+template <int D, bool BATCH>
 __global__ __launch_bounds__(256) void k_transpose_bits(const uint32_t* __restrict__ bits,
                                                         uint32_t* __restrict__ tb,
                                                         uint32_t* __restrict__ count, Geom g,
-                                                        int is_seed) {
+                                                        int is_seed, const BlockArgs* __restrict__ bt) {
     __shared__ uint32_t red[4];
+    if constexpr (BATCH) {
+        const BlockArgs& a = bt[blockIdx.z];
+        if (is_seed && a.seed == nullptr) return;  // resident seed: nothing to transpose
+        g = a.g;
+        bits = is_seed ? a.seed : a.key;
+        tb = is_seed ? a.tbs : a.tbk;
+        count = a.counts + (is_seed ? POP_SLOTS : 0);
+    }
     const uint32_t total = g.M * g.N2;
     uint32_t pc = 0;
 #pragma unroll
@@ -113,12 +124,20 @@ __global__ __launch_bounds__(256) void k_transpose_bits(const uint32_t* __restri
 // The class sum over v is four 16-entry table lookups per operand half (bits of tb).
 // Plan 16 x 32 for M = 512: stage 1 computed in registers from the bit masks, stage 2 written
 // straight to Y (thread t: column t mod 16, butterfly t / 16, rows k1' = j + 16 r).
-template <int LGM, int D>
+template <int LGM, int D, bool BATCH>
 __global__ __launch_bounds__(FWD_NT) void k_fwd_cols(const uint32_t* __restrict__ tb,
                                                      float2* __restrict__ Y, Geom g,
                                                      const float2* __restrict__ tw14,
                                                      const float2* __restrict__ twlo,
-                                                     const float2* __restrict__ twhi) {
+                                                     const float2* __restrict__ twhi,
+                                                     const BlockArgs* __restrict__ bt, int is_seed) {
+    if constexpr (BATCH) {
+        const BlockArgs& a = bt[blockIdx.z];
+        if (is_seed && a.seed == nullptr) return;
+        g = a.g;
+        tb = is_seed ? a.tbs : a.tbk;
+        Y = is_seed ? a.Ys : a.Yk;
+    }
     constexpr int M = 1 << LGM;
     constexpr int NT = FWD_NT;
     constexpr int SEQS = seq_stride(M);
@@ -261,11 +280,18 @@ __device__ __forceinline__ RowMap row_map(uint32_t w, uint32_t N1, int tid) {
     return m;
 }
 
-template <int MODE>
+template <int MODE, bool BATCH>
 __global__ __launch_bounds__(ROWS_NT) void k_rows(float2* __restrict__ Yk, float2* __restrict__ Sst,
                                                   Geom g, const float2* __restrict__ tw14,
                                                   const float2* __restrict__ twlo,
-                                                  const float2* __restrict__ twhi) {
+                                                  const float2* __restrict__ twhi,
+                                                  const BlockArgs* __restrict__ bt) {
+    if constexpr (BATCH) {
+        const BlockArgs& a = bt[blockIdx.z];
+        g = a.g;
+        Yk = a.Yk;
+        Sst = (MODE == ROWS_STATIC) ? const_cast<float2*>(a.sspec) : a.Ys;
+    }
     constexpr int N2 = 1 << ROW_LOG, NT = ROWS_NT, SEQS = seq_stride(N2);
     static_assert(N2 == 8192 && NT == 512, "row pass is written for 8192-point rows, 512 threads");
     static_assert(2 * N2 == TW14, "row split twiddle assumes N2 = 2^13");
@@ -454,11 +480,20 @@ __global__ __launch_bounds__(ROWS_NT) void k_rows(float2* __restrict__ Yk, float
 // unnormalised inverse, so c[2t] = Re G / N and c[2t+1] = -Im G / N at t = t2 + N2*t1.
 // Writes one nibble (4 conv bits, MSB first) per (column pair, t1) inside the output window, in
 // column-pair-major order nibT[pair * N1 + t1] so that each workgroup's writes are contiguous.
-template <int LG1>
+template <int LG1, bool BATCH>
 __global__ __launch_bounds__(inv_cols_threads(LG1)) void k_inv_cols(
     const float2* __restrict__ Rp, uint8_t* __restrict__ nibT, Geom g,
     const uint32_t* __restrict__ wk_cnt, const uint32_t* __restrict__ ws_cnt,
-    const float2* __restrict__ tw14, float* __restrict__ margin) {
+    const float2* __restrict__ tw14, float* __restrict__ margin, const BlockArgs* __restrict__ bt) {
+    if constexpr (BATCH) {
+        const BlockArgs& a = bt[blockIdx.z];
+        g = a.g;
+        Rp = a.Yk;
+        nibT = a.nib;
+        wk_cnt = a.counts;
+        ws_cnt = a.seed ? a.counts + POP_SLOTS : a.ws;
+        margin = a.margin;
+    }
     constexpr int N1 = 1 << LG1;
     constexpr int W = inv_cols_width(LG1);
     constexpr int NT = inv_cols_threads(LG1);
@@ -546,10 +581,19 @@ __global__ __launch_bounds__(inv_cols_threads(LG1)) void k_inv_cols(
 // [out_off + 32w, +32), XOR key_rest word w ((key[h/32+w] << 1) | (key[h/32+w+1] >> 31),
 // key2StartRest PrivacyAmplification.cu:873-878), byte-swapped for the OutBytes layout
 // (ToBinaryArray, PrivacyAmplification.cu:757-762).
+template <bool BATCH>
 __global__ __launch_bounds__(256) void k_pack(const uint8_t* __restrict__ nibT,
                                               const uint32_t* __restrict__ key,
-                                              uint32_t* __restrict__ out, Geom g) {
+                                              uint32_t* __restrict__ out, Geom g,
+                                              const BlockArgs* __restrict__ bt) {
     __shared__ uint8_t tile[PACK_T1][PACK_G + 4];
+    if constexpr (BATCH) {  // grid.y covers the largest window of the launch; rows past this
+        const BlockArgs& a = bt[blockIdx.z];  // block's window load nothing and store nothing
+        g = a.g;
+        nibT = a.nib;
+        key = a.key;
+        out = a.out;
+    }
     const uint32_t P2 = g.N2 >> 1;
     const uint32_t q_lo = g.out_off >> 2, q_hi = (g.out_off + g.out_bits) >> 2;
     const uint32_t row0 = ((q_lo / P2) & ~3u) + blockIdx.y * PACK_T1;  // 4-aligned for 32-bit reads
@@ -581,12 +625,23 @@ __global__ __launch_bounds__(256) void k_pack(const uint8_t* __restrict__ nibT,
 // The whole block in one workgroup: N = 2^LGN <= 8192 complex points per operand in LDS.
 // sspec != nullptr: static seed (precomputed split spectrum, N+1 values, popcount *ws_ptr).
 // SEEDPREP: only computes the seed's split spectrum into sspec_out and its popcount.
-template <int LGN, bool SEEDPREP>
+template <int LGN, bool SEEDPREP, bool BATCH>
 __global__ __launch_bounds__(small_threads(LGN)) void k_small(
     const uint32_t* __restrict__ key, const uint32_t* __restrict__ seed,
     const float2* __restrict__ sspec, const uint32_t* __restrict__ ws_ptr,
     uint32_t* __restrict__ out, Geom g, const float2* __restrict__ tw14,
-    float* __restrict__ margin, float2* __restrict__ sspec_out, uint32_t* __restrict__ ws_out) {
+    float* __restrict__ margin, float2* __restrict__ sspec_out, uint32_t* __restrict__ ws_out,
+    const BlockArgs* __restrict__ bt) {
+    if constexpr (BATCH) {
+        const BlockArgs& a = bt[blockIdx.z];
+        g = a.g;
+        key = a.key;
+        seed = a.seed;
+        sspec = a.seed ? nullptr : a.sspec;
+        ws_ptr = a.ws;
+        out = a.out;
+        margin = a.margin;
+    }
     constexpr int N = 1 << LGN, L = 2 * N;
     constexpr int NT = small_threads(LGN);
     constexpr int SEQS = seq_stride(N);
@@ -713,14 +768,14 @@ namespace {
 template <int LGM, int D>
 hipError_t launch_fwd_cols_t(const uint32_t* tb, float2* Y, const Geom& g, const Tables& t, hipStream_t s) {
     const uint32_t grid = (g.N2 / 16u) * (uint32_t)D;
-    k_fwd_cols<LGM, D><<<grid, FWD_NT, 0, s>>>(tb, Y, g, t.tw14, t.lo, t.hi);
+    k_fwd_cols<LGM, D, false><<<grid, FWD_NT, 0, s>>>(tb, Y, g, t.tw14, t.lo, t.hi, nullptr, 0);
     return hipGetLastError();
 }
 template <int LG1>
 hipError_t launch_inv_cols_t(const float2* R, uint8_t* nib, const Geom& g, const uint32_t* wk,
                              const uint32_t* ws, const Tables& t, float* margin, hipStream_t s) {
     const uint32_t grid = g.N2 / (uint32_t)inv_cols_width(LG1);
-    k_inv_cols<LG1><<<grid, inv_cols_threads(LG1), 0, s>>>(R, nib, g, wk, ws, t.tw14, margin);
+    k_inv_cols<LG1, false><<<grid, inv_cols_threads(LG1), 0, s>>>(R, nib, g, wk, ws, t.tw14, margin, nullptr);
     return hipGetLastError();
 }
 template <int D>
@@ -728,7 +783,7 @@ hipError_t launch_transpose_t(const uint32_t* bits, uint32_t* tb, uint32_t* coun
                               int is_seed, hipStream_t s) {
     const uint32_t total = g.M * g.N2;
     const uint32_t per_block = 256u * TB_PER;
-    k_transpose_bits<D><<<(total + per_block - 1u) / per_block, 256, 0, s>>>(bits, tb, count, g, is_seed);
+    k_transpose_bits<D, false><<<(total + per_block - 1u) / per_block, 256, 0, s>>>(bits, tb, count, g, is_seed, nullptr);
     return hipGetLastError();
 }
 template <int LGN>
@@ -736,17 +791,17 @@ hipError_t launch_small_t(const uint32_t* key, const uint32_t* seed, const float
                           const uint32_t* ws_ptr, uint32_t* out, const Geom& g, const Tables& t,
                           float* margin, float2* sspec_out, uint32_t* ws_out, hipStream_t s) {
     if (sspec_out)
-        k_small<LGN, true><<<1, small_threads(LGN), 0, s>>>(key, seed, sspec, ws_ptr, out, g, t.tw14,
-                                                           margin, sspec_out, ws_out);
+        k_small<LGN, true, false><<<1, small_threads(LGN), 0, s>>>(key, seed, sspec, ws_ptr, out, g, t.tw14,
+                                                                  margin, sspec_out, ws_out, nullptr);
     else
-        k_small<LGN, false><<<1, small_threads(LGN), 0, s>>>(key, seed, sspec, ws_ptr, out, g, t.tw14,
-                                                            margin, sspec_out, ws_out);
+        k_small<LGN, false, false><<<1, small_threads(LGN), 0, s>>>(key, seed, sspec, ws_ptr, out, g, t.tw14,
+                                                                   margin, sspec_out, ws_out, nullptr);
     return hipGetLastError();
 }
 }  // namespace
 
 hipError_t launch_zero_u32(uint32_t* p, uint32_t count, hipStream_t s) {
-    k_zero_u32<<<1, 256, 0, s>>>(p, count);
+    k_zero_u32<false><<<1, 256, 0, s>>>(p, count, nullptr);
     return hipGetLastError();
 }
 
@@ -785,9 +840,9 @@ hipError_t launch_rows(int mode, float2* Yk, float2* S, const Geom& g, const Tab
     const uint32_t items = g.N1 / 2u;
     const uint32_t grid = items < (uint32_t)cus ? items : (uint32_t)cus;  // one workgroup per CU
     switch (mode) {
-    case ROWS_STATIC: k_rows<ROWS_STATIC><<<grid, ROWS_NT, 0, s>>>(Yk, S, g, t.tw14, t.lo, t.hi); break;
-    case ROWS_DYN: k_rows<ROWS_DYN><<<grid, ROWS_NT, 0, s>>>(Yk, S, g, t.tw14, t.lo, t.hi); break;
-    case ROWS_SEEDPREP: k_rows<ROWS_SEEDPREP><<<grid, ROWS_NT, 0, s>>>(Yk, S, g, t.tw14, t.lo, t.hi); break;
+    case ROWS_STATIC: k_rows<ROWS_STATIC, false><<<grid, ROWS_NT, 0, s>>>(Yk, S, g, t.tw14, t.lo, t.hi, nullptr); break;
+    case ROWS_DYN: k_rows<ROWS_DYN, false><<<grid, ROWS_NT, 0, s>>>(Yk, S, g, t.tw14, t.lo, t.hi, nullptr); break;
+    case ROWS_SEEDPREP: k_rows<ROWS_SEEDPREP, false><<<grid, ROWS_NT, 0, s>>>(Yk, S, g, t.tw14, t.lo, t.hi, nullptr); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -818,7 +873,7 @@ hipError_t launch_pack(const uint8_t* nibT, const uint32_t* key, uint32_t* out, 
     const uint32_t q_lo = g.out_off >> 2, q_hi = (g.out_off + g.out_bits) >> 2;
     const uint32_t rows = (q_hi - 1u) / P2 - ((q_lo / P2) & ~3u) + 1u;
     dim3 grid(P2 / PACK_G, (rows + PACK_T1 - 1u) / PACK_T1);
-    k_pack<<<grid, 256, 0, s>>>(nibT, key, out, g);
+    k_pack<false><<<grid, 256, 0, s>>>(nibT, key, out, g, nullptr);
     return hipGetLastError();
 }
 
@@ -835,6 +890,133 @@ hipError_t launch_small(const uint32_t* key, const uint32_t* seed, const float2*
     case 11: return launch_small_t<11>(key, seed, sspec, ws_ptr, out, g, t, margin, sspec_out, ws_out, s);
     case 12: return launch_small_t<12>(key, seed, sspec, ws_ptr, out, g, t, margin, sspec_out, ws_out, s);
     case 13: return launch_small_t<13>(key, seed, sspec, ws_ptr, out, g, t, margin, sspec_out, ws_out, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// ---------------------------------------------------------------- batched launchers ----
+// `count` blocks of one transform length; block b's geometry and buffers are bt[b] (device
+// memory) and g0 is any block's geometry (only its L-derived fields are used here).
+uint32_t pack_tiles(const Geom& g) {
+    const uint32_t P2 = g.N2 >> 1;
+    const uint32_t q_lo = g.out_off >> 2, q_hi = (g.out_off + g.out_bits) >> 2;
+    const uint32_t rows = (q_hi - 1u) / P2 - ((q_lo / P2) & ~3u) + 1u;
+    return (rows + PACK_T1 - 1u) / PACK_T1;
+}
+
+hipError_t launch_batch_zero(const BlockArgs* bt, uint32_t count, hipStream_t s) {
+    k_zero_u32<true><<<dim3(1, 1, count), 256, 0, s>>>(nullptr, 2 * POP_SLOTS, bt);
+    return hipGetLastError();
+}
+
+namespace {
+template <int D>
+hipError_t batch_transpose_t(const BlockArgs* bt, uint32_t count, const Geom& g0, int is_seed, hipStream_t s) {
+    const uint32_t total = g0.M * g0.N2, per_block = 256u * TB_PER;
+    k_transpose_bits<D, true><<<dim3((total + per_block - 1u) / per_block, 1, count), 256, 0, s>>>(
+        nullptr, nullptr, nullptr, g0, is_seed, bt);
+    return hipGetLastError();
+}
+template <int LGM, int D>
+hipError_t batch_fwd_cols_t(const BlockArgs* bt, uint32_t count, const Geom& g0, const Tables& t, int is_seed,
+                            hipStream_t s) {
+    k_fwd_cols<LGM, D, true><<<dim3((g0.N2 / 16u) * (uint32_t)D, 1, count), FWD_NT, 0, s>>>(
+        nullptr, nullptr, g0, t.tw14, t.lo, t.hi, bt, is_seed);
+    return hipGetLastError();
+}
+template <int LG1>
+hipError_t batch_inv_cols_t(const BlockArgs* bt, uint32_t count, const Geom& g0, const Tables& t, hipStream_t s) {
+    k_inv_cols<LG1, true><<<dim3(g0.N2 / (uint32_t)inv_cols_width(LG1), 1, count), inv_cols_threads(LG1), 0, s>>>(
+        nullptr, nullptr, g0, nullptr, nullptr, t.tw14, nullptr, bt);
+    return hipGetLastError();
+}
+template <int LGN>
+hipError_t batch_small_t(const BlockArgs* bt, uint32_t count, const Geom& g0, const Tables& t, hipStream_t s) {
+    k_small<LGN, false, true><<<dim3(1, 1, count), small_threads(LGN), 0, s>>>(
+        nullptr, nullptr, nullptr, nullptr, nullptr, g0, t.tw14, nullptr, nullptr, nullptr, bt);
+    return hipGetLastError();
+}
+}  // namespace
+
+hipError_t launch_batch_transpose(const BlockArgs* bt, uint32_t count, const Geom& g0, int is_seed, hipStream_t s) {
+    switch (g0.D) {
+    case 1: return batch_transpose_t<1>(bt, count, g0, is_seed, s);
+    case 2: return batch_transpose_t<2>(bt, count, g0, is_seed, s);
+    case 4: return batch_transpose_t<4>(bt, count, g0, is_seed, s);
+    case 8: return batch_transpose_t<8>(bt, count, g0, is_seed, s);
+    case 16: return batch_transpose_t<16>(bt, count, g0, is_seed, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_batch_fwd_cols(const BlockArgs* bt, uint32_t count, const Geom& g0, const Tables& t, int is_seed,
+                                 hipStream_t s) {
+    switch (g0.lg1) {
+    case 1: return batch_fwd_cols_t<1, 1>(bt, count, g0, t, is_seed, s);
+    case 2: return batch_fwd_cols_t<2, 1>(bt, count, g0, t, is_seed, s);
+    case 3: return batch_fwd_cols_t<3, 1>(bt, count, g0, t, is_seed, s);
+    case 4: return batch_fwd_cols_t<4, 1>(bt, count, g0, t, is_seed, s);
+    case 5: return batch_fwd_cols_t<5, 1>(bt, count, g0, t, is_seed, s);
+    case 6: return batch_fwd_cols_t<6, 1>(bt, count, g0, t, is_seed, s);
+    case 7: return batch_fwd_cols_t<7, 1>(bt, count, g0, t, is_seed, s);
+    case 8: return batch_fwd_cols_t<8, 1>(bt, count, g0, t, is_seed, s);
+    case 9: return batch_fwd_cols_t<9, 1>(bt, count, g0, t, is_seed, s);
+    case 10: return batch_fwd_cols_t<9, 2>(bt, count, g0, t, is_seed, s);
+    case 11: return batch_fwd_cols_t<9, 4>(bt, count, g0, t, is_seed, s);
+    case 12: return batch_fwd_cols_t<9, 8>(bt, count, g0, t, is_seed, s);
+    case 13: return batch_fwd_cols_t<9, 16>(bt, count, g0, t, is_seed, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_batch_rows(int mode, const BlockArgs* bt, uint32_t count, const Geom& g0, const Tables& t,
+                             uint32_t grid_x, hipStream_t s) {
+    const uint32_t items = g0.N1 / 2u;
+    if (grid_x == 0 || grid_x > items) grid_x = items;
+    const dim3 grid(grid_x, 1, count);
+    switch (mode) {
+    case ROWS_STATIC: k_rows<ROWS_STATIC, true><<<grid, ROWS_NT, 0, s>>>(nullptr, nullptr, g0, t.tw14, t.lo, t.hi, bt); break;
+    case ROWS_DYN: k_rows<ROWS_DYN, true><<<grid, ROWS_NT, 0, s>>>(nullptr, nullptr, g0, t.tw14, t.lo, t.hi, bt); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_inv_cols(const BlockArgs* bt, uint32_t count, const Geom& g0, const Tables& t, hipStream_t s) {
+    switch (g0.lg1) {
+    case 1: return batch_inv_cols_t<1>(bt, count, g0, t, s);
+    case 2: return batch_inv_cols_t<2>(bt, count, g0, t, s);
+    case 3: return batch_inv_cols_t<3>(bt, count, g0, t, s);
+    case 4: return batch_inv_cols_t<4>(bt, count, g0, t, s);
+    case 5: return batch_inv_cols_t<5>(bt, count, g0, t, s);
+    case 6: return batch_inv_cols_t<6>(bt, count, g0, t, s);
+    case 7: return batch_inv_cols_t<7>(bt, count, g0, t, s);
+    case 8: return batch_inv_cols_t<8>(bt, count, g0, t, s);
+    case 9: return batch_inv_cols_t<9>(bt, count, g0, t, s);
+    case 10: return batch_inv_cols_t<10>(bt, count, g0, t, s);
+    case 11: return batch_inv_cols_t<11>(bt, count, g0, t, s);
+    case 12: return batch_inv_cols_t<12>(bt, count, g0, t, s);
+    case 13: return batch_inv_cols_t<13>(bt, count, g0, t, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_batch_pack(const BlockArgs* bt, uint32_t count, const Geom& g0, uint32_t tiles_y, hipStream_t s) {
+    k_pack<true><<<dim3((g0.N2 >> 1) / PACK_G, tiles_y, count), 256, 0, s>>>(nullptr, nullptr, nullptr, g0, bt);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_small(const BlockArgs* bt, uint32_t count, const Geom& g0, const Tables& t, hipStream_t s) {
+    switch (g0.lgN) {
+    case 5: return batch_small_t<5>(bt, count, g0, t, s);
+    case 6: return batch_small_t<6>(bt, count, g0, t, s);
+    case 7: return batch_small_t<7>(bt, count, g0, t, s);
+    case 8: return batch_small_t<8>(bt, count, g0, t, s);
+    case 9: return batch_small_t<9>(bt, count, g0, t, s);
+    case 10: return batch_small_t<10>(bt, count, g0, t, s);
+    case 11: return batch_small_t<11>(bt, count, g0, t, s);
+    case 12: return batch_small_t<12>(bt, count, g0, t, s);
+    case 13: return batch_small_t<13>(bt, count, g0, t, s);
     default: return hipErrorInvalidValue;
     }
 }

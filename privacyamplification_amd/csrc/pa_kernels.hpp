@@ -49,4 +49,18 @@ hipError_t launch_small(const uint32_t* key, const uint32_t* seed, const float2*
                         const uint32_t* ws_ptr, uint32_t* out, const Geom& g, const Tables& t,
                         float* margin, float2* sspec_out, uint32_t* ws_out, hipStream_t s);
 
+// Batched forms: `count` blocks of one transform length in one launch each (grid z = block),
+// block b's geometry and buffers in bt[b] (device memory); g0 is any block's geometry.
+uint32_t pack_tiles(const Geom& g);  // grid.y the pack stage needs for this block's window
+hipError_t launch_batch_zero(const BlockArgs* bt, uint32_t count, hipStream_t s);
+hipError_t launch_batch_transpose(const BlockArgs* bt, uint32_t count, const Geom& g0, int is_seed, hipStream_t s);
+hipError_t launch_batch_fwd_cols(const BlockArgs* bt, uint32_t count, const Geom& g0, const Tables& t, int is_seed,
+                                 hipStream_t s);
+// grid_x workgroups per block walk that block's row pairs (0 = one per row pair).
+hipError_t launch_batch_rows(int mode, const BlockArgs* bt, uint32_t count, const Geom& g0, const Tables& t,
+                             uint32_t grid_x, hipStream_t s);
+hipError_t launch_batch_inv_cols(const BlockArgs* bt, uint32_t count, const Geom& g0, const Tables& t, hipStream_t s);
+hipError_t launch_batch_pack(const BlockArgs* bt, uint32_t count, const Geom& g0, uint32_t tiles_y, hipStream_t s);
+hipError_t launch_batch_small(const BlockArgs* bt, uint32_t count, const Geom& g0, const Tables& t, hipStream_t s);
+
 }  // namespace pa

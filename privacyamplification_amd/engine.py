@@ -49,6 +49,25 @@ class PaConfig(ctypes.Structure):
     ]
 
 
+class PaBlock(ctypes.Structure):
+    """One block of a batch (pa_block): device pointers for batch_device, host for run_batch."""
+    _fields_ = [
+        ("key", ctypes.c_void_p),
+        ("seed", ctypes.c_void_p),
+        ("seed_id", ctypes.c_uint64),
+        ("n", ctypes.c_uint32),
+        ("m", ctypes.c_uint32),
+        ("flags", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+        ("out", ctypes.c_void_p),
+        ("margin", ctypes.c_void_p),
+    ]
+
+
+BATCH_GRAPH = 1
+BATCH_TUNE = 2
+
+
 def library_path():
     return _LIB
 
@@ -70,6 +89,13 @@ def load_library():
     if not _LIB.exists():
         raise RuntimeError(f"{_LIB} is missing: build it with privacyamplification_amd.build() "
                            "(there is no CPU fallback)")
+    # One HIP runtime per process: torch ships its own libamdhip64 (same soname).  Loading torch
+    # first makes this library bind to that runtime, so device pointers and stream handles from
+    # torch are valid here; loaded the other way round, torch would start a second runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(str(_LIB))
     u32p, u8p, vp = ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint8), ctypes.c_void_p
     eng = ctypes.c_void_p
@@ -89,6 +115,8 @@ def load_library():
                                   vp, vp]
     L.pa_block_device_seeded.argtypes = [eng, vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
                                          ctypes.c_uint32, vp, vp, vp]
+    L.pa_batch_device.argtypes = [eng, ctypes.POINTER(PaBlock), ctypes.c_uint32, ctypes.c_uint32, vp]
+    L.pa_run_batch.argtypes = [eng, ctypes.POINTER(PaBlock), ctypes.c_uint32, ctypes.c_uint32]
     L.pa_stage_timing.argtypes = [eng, ctypes.c_int]
     L.pa_stage_times.argtypes = [eng, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_double),
                                  ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]
@@ -100,7 +128,8 @@ def load_library():
     L.pa_version.restype = ctypes.c_char_p
     for name in ("pa_config_default", "pa_create", "pa_destroy", "pa_seed_put", "pa_seed_drop",
                  "pa_run_block", "pa_run_block_seeded", "pa_submit", "pa_wait", "pa_block_device",
-                 "pa_block_device_seeded", "pa_synth_bits", "pa_stage_timing", "pa_stage_times"):
+                 "pa_block_device_seeded", "pa_synth_bits", "pa_stage_timing", "pa_stage_times",
+                 "pa_batch_device", "pa_run_batch"):
         getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L
@@ -220,6 +249,38 @@ class Engine:
     def block_device_seeded(self, d_key, sid, n, m, flags, d_out, stream=None, d_margin=None):
         _check(self._L.pa_block_device_seeded(self._h, d_key, sid, n, m, flags, d_out, stream, d_margin),
                "pa_block_device_seeded")
+
+    def run_batch(self, blocks, batch_flags=0):
+        """Host-buffer batch (pa_run_batch).  blocks: dicts with key, seed (or None with sid), n, m,
+        and optional do_xor_key_rest / do_compress / sid.  Returns the amp_out arrays in order."""
+        arr = (PaBlock * len(blocks))()
+        keep, outs = [], []
+        for i, b in enumerate(blocks):
+            fl = self.flags(b.get("do_xor_key_rest", True), b.get("do_compress", True))
+            k, _ = _u32(b["key"])
+            keep.append(k)
+            arr[i].key = k.ctypes.data
+            if b.get("seed") is not None:
+                sd, _ = _u32(b["seed"])
+                keep.append(sd)
+                arr[i].seed = sd.ctypes.data
+            arr[i].seed_id = b.get("sid", 0)
+            arr[i].n, arr[i].m, arr[i].flags = b["n"], b["m"], fl
+            o = np.zeros(out_bytes(b["n"], b["m"], fl & COMPRESS) + 8, dtype=np.uint8)
+            outs.append(o)
+            arr[i].out = o.ctypes.data
+        _check(self._L.pa_run_batch(self._h, arr, len(blocks), batch_flags), "pa_run_batch")
+        return [o[: out_bytes(b["n"], b["m"], self.flags(b.get("do_xor_key_rest", True),
+                                                         b.get("do_compress", True)) & COMPRESS)]
+                for o, b in zip(outs, blocks)]
+
+    def batch_device(self, blocks, batch_flags=0, stream=None):
+        """Device batch (pa_batch_device): blocks is a PaBlock array (or a list of PaBlock)."""
+        if not isinstance(blocks, ctypes.Array):
+            arr = (PaBlock * len(blocks))(*blocks)
+        else:
+            arr = blocks
+        _check(self._L.pa_batch_device(self._h, arr, len(arr), batch_flags, stream), "pa_batch_device")
 
     def stage_timing(self, enable=True):
         _check(self._L.pa_stage_timing(self._h, 1 if enable else 0), "pa_stage_timing")
