@@ -141,6 +141,39 @@ int pa_batch_device(pa_engine* eng, const pa_block* blocks, uint32_t count, uint
 /* Same with host buffers; returns when every output has been written. */
 int pa_run_batch(pa_engine* eng, const pa_block* blocks, uint32_t count, uint32_t batch_flags);
 
+/* Pinned host memory for key, seed and output buffers: the counterpart of the reference's
+ * cudaMallocHost'd input and output block caches (PrivacyAmplification.cu:1535-1551).  Blocks
+ * whose buffers lie in such memory are copied by DMA straight from and to it; other host memory
+ * is staged through the engine's own pinned buffers. */
+int pa_host_alloc(pa_engine* eng, uint64_t bytes, void** ptr);
+int pa_host_free(pa_engine* eng, void* ptr);
+
+/* Serving: a scheduler thread inside the engine admits submitted blocks into batches while
+ * earlier batches are still on the GPU (continuous batching), within an HBM budget for in-flight
+ * blocks.  The native form of the reference's receive -> mainloop -> send pipeline with its input
+ * and output block caches (input_blocks_to_cache / output_blocks_to_cache, spin-wait rings,
+ * PrivacyAmplification.cu:202-207, 1686-1717, 2098-2103). */
+typedef struct {
+    uint32_t slots;            /* batches in flight at once (default 3) */
+    uint32_t max_batch;        /* blocks per batch (default 256) */
+    uint64_t hbm_budget_bytes; /* device memory for in-flight blocks, split evenly over the slots
+                                  (default 16 GiB); a block larger than one slot's share fails
+                                  with PA_ERR_NOMEM */
+    uint32_t batch_flags;      /* PA_BATCH_* for every batch (default 0) */
+    uint32_t reserved;
+} pa_serve_config;
+
+/* Starts the scheduler (cfg NULL: defaults).  At most one server per engine. */
+int pa_serve_start(pa_engine* eng, const pa_serve_config* cfg);
+/* Queues one block (host pointers, which must stay valid until pa_serve_wait returns for it);
+ * returns at once with a ticket.  Resident seeds must not be dropped while blocks use them. */
+int pa_serve_submit(pa_engine* eng, const pa_block* block, pa_ticket* ticket);
+/* Waits for a block (timeout_ns < 0: forever).  On success *latency_ns (may be NULL) receives the
+ * time from pa_serve_submit to the output being written; returns the block's own status. */
+int pa_serve_wait(pa_engine* eng, pa_ticket ticket, int64_t timeout_ns, uint64_t* latency_ns);
+/* Drains every queued and in-flight block, then stops the scheduler. */
+int pa_serve_stop(pa_engine* eng);
+
 /* Per-stage device timing, the counterpart of the reference's STOPWATCH stage timers
  * (PrivacyAmplification.cu:146-166, 2053-2074).  When enabled, the engine records HIP events
  * around every kernel it launches (on the launch stream).  pa_stage_times synchronises, then

@@ -261,51 +261,72 @@ static void ntt_core(uint32_t* a, uint32_t n, const uint32_t* tw /* w^j, j < n/2
         }
     }
 }
-int pa_or_conv_counts(const uint32_t* ks, uint32_t kbits, const uint32_t* seed, uint32_t n,
-                      uint32_t nout, uint32_t* counts) {
-    if (n < 2 || (n & (n - 1)) || n > (1u << 27) || kbits > n || nout > n) return -1;
-    ntt_init();
-    uint32_t* a = (uint32_t*)malloc((size_t)n * 4);
-    uint32_t* b = (uint32_t*)malloc((size_t)n * 4);
-    uint32_t* tw = (uint32_t*)malloc((size_t)(n / 2) * 4 + 4);
-    if (!a || !b || !tw) { free(a); free(b); free(tw); return -1; }
-    uint32_t one = to_mont(1);
-#pragma omp parallel for schedule(static)
-    for (int64_t j = 0; j < (int64_t)n; ++j) {
-        a[j] = ((uint32_t)j < kbits && refbit(ks, (uint64_t)j)) ? one : 0u;
-        b[j] = refbit(seed, (uint64_t)j) ? one : 0u;
-    }
+/* a <- a (*) b cyclically over Lc (a power of two, Lc <= 2^27), Montgomery form in and out. */
+static int ntt_cyclic(uint32_t* a, uint32_t* b, uint32_t Lc) {
+    uint32_t* tw = (uint32_t*)malloc((size_t)(Lc / 2) * 4 + 4);
+    if (!tw) return -1;
     uint32_t g = to_mont(31);
-    uint32_t w = mpow(g, (uint64_t)(NTT_P - 1) / n); /* primitive n-th root */
+    uint32_t w = mpow(g, (uint64_t)(NTT_P - 1) / Lc); /* primitive Lc-th root */
     /* tw[j] = w^j in blocks so that the table is filled in parallel */
     {
-        uint32_t half = n / 2, chunk = 4096;
+        uint32_t half = Lc / 2, chunk = 4096;
 #pragma omp parallel for schedule(static)
         for (int64_t c0 = 0; c0 < (int64_t)half; c0 += chunk) {
             uint32_t x = mpow(w, (uint64_t)c0);
             for (uint32_t j = (uint32_t)c0; j < half && j < (uint32_t)c0 + chunk; ++j) { tw[j] = x; x = mmul(x, w); }
         }
     }
-    ntt_core(a, n, tw, 0);
-    ntt_core(b, n, tw, 0);
+    ntt_core(a, Lc, tw, 0);
+    ntt_core(b, Lc, tw, 0);
 #pragma omp parallel for schedule(static)
-    for (int64_t j = 0; j < (int64_t)n; ++j) a[j] = mmul(a[j], b[j]);
+    for (int64_t j = 0; j < (int64_t)Lc; ++j) a[j] = mmul(a[j], b[j]);
     /* inverse uses w^-1 */
     {
-        uint32_t winv = mpow(w, (uint64_t)n - 1);
-        uint32_t half = n / 2, chunk = 4096;
+        uint32_t winv = mpow(w, (uint64_t)Lc - 1);
+        uint32_t half = Lc / 2, chunk = 4096;
 #pragma omp parallel for schedule(static)
         for (int64_t c0 = 0; c0 < (int64_t)half; c0 += chunk) {
             uint32_t x = mpow(winv, (uint64_t)c0);
             for (uint32_t j = (uint32_t)c0; j < half && j < (uint32_t)c0 + chunk; ++j) { tw[j] = x; x = mmul(x, winv); }
         }
     }
-    ntt_core(a, n, tw, 1);
-    uint32_t ninv = mpow(to_mont(n), (uint64_t)NTT_P - 2);
+    ntt_core(a, Lc, tw, 1);
+    uint32_t ninv = mpow(to_mont(Lc), (uint64_t)NTT_P - 2);
 #pragma omp parallel for schedule(static)
-    for (int64_t j = 0; j < (int64_t)nout; ++j) counts[j] = from_mont(mmul(a[j], ninv));
-    free(a); free(b); free(tw);
+    for (int64_t j = 0; j < (int64_t)Lc; ++j) a[j] = mmul(a[j], ninv);
+    free(tw);
     return 0;
+}
+
+/* counts[p] = sum_{j < kbits} x[j] s[(p - j) mod n] exactly, p < nout.  Power-of-two n: one cyclic
+ * NTT of length n.  Other n: the linear convolution of x with the seed repeated twice (length 2n)
+ * read at p + n, on an NTT of length >= kbits + 2n so nothing wraps.  Exact while every sum stays
+ * below the prime (kbits < 2^31) and the length fits its 2^27-th roots. */
+int pa_or_conv_counts(const uint32_t* ks, uint32_t kbits, const uint32_t* seed, uint32_t n,
+                      uint32_t nout, uint32_t* counts) {
+    if (n < 2 || kbits > n || nout > n) return -1;
+    const int pow2 = (n & (n - 1)) == 0;
+    uint64_t Lc = 1;
+    while (Lc < (pow2 ? (uint64_t)n : (uint64_t)kbits + 2ull * n)) Lc <<= 1;
+    if (Lc > (1ull << 27)) return -1;
+    ntt_init();
+    uint32_t* a = (uint32_t*)calloc((size_t)Lc, 4);
+    uint32_t* b = (uint32_t*)calloc((size_t)Lc, 4);
+    if (!a || !b) { free(a); free(b); return -1; }
+    uint32_t one = to_mont(1);
+    const uint64_t bl = pow2 ? n : 2ull * n;
+#pragma omp parallel for schedule(static)
+    for (int64_t j = 0; j < (int64_t)kbits; ++j) a[j] = refbit(ks, (uint64_t)j) ? one : 0u;
+#pragma omp parallel for schedule(static)
+    for (int64_t t = 0; t < (int64_t)bl; ++t) b[t] = refbit(seed, (uint64_t)t % n) ? one : 0u;
+    int rc = ntt_cyclic(a, b, (uint32_t)Lc);
+    if (rc == 0) {
+        const uint64_t base = pow2 ? 0 : n;
+#pragma omp parallel for schedule(static)
+        for (int64_t p = 0; p < (int64_t)nout; ++p) counts[p] = from_mont(a[base + (uint64_t)p]);
+    }
+    free(a); free(b);
+    return rc;
 }
 
 static int conv_ntt(const uint32_t* ks, uint32_t kbits, const uint32_t* seed, uint32_t n,

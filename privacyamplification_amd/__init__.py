@@ -13,6 +13,7 @@ from .engine import (  # noqa: F401
     Engine,
     EngineError,
     PaBlock,
+    PaServeConfig,
     build,
     library_path,
     load_library,

@@ -15,6 +15,8 @@
 #include <map>
 #include <memory>
 #include <algorithm>
+#include <condition_variable>
+#include <deque>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -136,6 +138,22 @@ struct Ticket {
     size_t out_bytes = 0;
 };
 
+// A batch in flight: its device arena (block table, scratch, staging), the pinned host copy of its
+// block table, fork streams for concurrent groups, and the graphs captured for its shapes.
+struct BatchSlot {
+    static constexpr int FORKS = 4;
+    hipStream_t stream = nullptr;              // the slot's own launch stream (serving)
+    hipStream_t fork[FORKS] = {};
+    hipEvent_t fork_ev = nullptr, join_ev[FORKS] = {};
+    uint8_t* arena = nullptr;                  // grow-only device arena
+    uint64_t arena_cap = 0;
+    pa::BlockArgs* htab = nullptr;             // pinned host copy of the block table
+    uint32_t htab_cap = 0;
+    hipEvent_t htab_free = nullptr;            // recorded after the last table upload
+    bool htab_pending = false;
+    std::map<std::string, hipGraphExec_t> graphs;
+};
+
 }  // namespace
 
 struct pa_engine {
@@ -159,17 +177,11 @@ struct pa_engine {
     double stage_ms[16] = {};
     uint32_t stage_count[16] = {};
     // batched launches
-    static constexpr int FORKS = 4;
-    hipStream_t fork[FORKS] = {};
-    hipEvent_t fork_ev = nullptr, join_ev[FORKS] = {};
-    uint8_t* arena = nullptr;                // grow-only device arena: table, scratch, staging
-    uint64_t arena_cap = 0;
-    pa::BlockArgs* htab = nullptr;           // pinned host copy of the block table
-    uint32_t htab_cap = 0;
-    hipEvent_t htab_free = nullptr;          // recorded after the last table upload
-    bool htab_pending = false;
-    std::map<std::string, hipGraphExec_t> graphs;
+    BatchSlot batch;                         // pa_batch_device / pa_run_batch
     std::map<std::string, uint32_t> tuned;   // row-pass grid per (L, seed mode, count)
+    struct Server* server = nullptr;         // pa_serve_*
+    std::map<uintptr_t, uint64_t> pinned;    // pa_host_alloc ranges: base -> bytes
+    std::mutex pin_mu;
 };
 
 namespace {
@@ -189,6 +201,37 @@ void dev_free(pa_engine* e, void* p, size_t bytes) {
     if (!p) return;
     (void)hipFree(p);
     e->used_bytes -= bytes;
+}
+
+// Streams and events of a batch slot (own_stream: a launch stream of its own, for serving).
+int slot_init(BatchSlot* sl, bool own_stream) {
+    if (own_stream) PA_HIP(hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking));
+    for (int i = 0; i < BatchSlot::FORKS; ++i) {
+        PA_HIP(hipStreamCreateWithFlags(&sl->fork[i], hipStreamNonBlocking));
+        PA_HIP(hipEventCreateWithFlags(&sl->join_ev[i], hipEventDisableTiming));
+    }
+    PA_HIP(hipEventCreateWithFlags(&sl->fork_ev, hipEventDisableTiming));
+    PA_HIP(hipEventCreateWithFlags(&sl->htab_free, hipEventDisableTiming));
+    return PA_OK;
+}
+
+void slot_release(pa_engine* e, BatchSlot* sl) {
+    if (sl->stream) (void)hipStreamSynchronize(sl->stream);
+    for (auto& kv : sl->graphs) (void)hipGraphExecDestroy(kv.second);
+    sl->graphs.clear();
+    for (int i = 0; i < BatchSlot::FORKS; ++i) {
+        if (sl->fork[i]) {
+            (void)hipStreamSynchronize(sl->fork[i]);
+            (void)hipStreamDestroy(sl->fork[i]);
+        }
+        if (sl->join_ev[i]) (void)hipEventDestroy(sl->join_ev[i]);
+    }
+    if (sl->fork_ev) (void)hipEventDestroy(sl->fork_ev);
+    if (sl->htab_free) (void)hipEventDestroy(sl->htab_free);
+    if (sl->arena) dev_free(e, sl->arena, sl->arena_cap);
+    if (sl->htab) (void)hipHostFree(sl->htab);
+    if (sl->stream) (void)hipStreamDestroy(sl->stream);
+    *sl = BatchSlot();
 }
 
 std::vector<float2> twiddles(uint64_t L, uint64_t count, uint64_t stride) {
@@ -376,13 +419,9 @@ int pa_create(const pa_config* cfg, pa_engine** out) {
     PA_HIP(hipSetDevice(e->cfg.device));
     PA_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
     PA_HIP(hipDeviceGetAttribute(&e->cus, hipDeviceAttributeMultiprocessorCount, e->cfg.device));
-    for (int i = 0; i < pa_engine::FORKS; ++i) {
-        PA_HIP(hipStreamCreateWithFlags(&e->fork[i], hipStreamNonBlocking));
-        PA_HIP(hipEventCreateWithFlags(&e->join_ev[i], hipEventDisableTiming));
-    }
-    PA_HIP(hipEventCreateWithFlags(&e->fork_ev, hipEventDisableTiming));
-    PA_HIP(hipEventCreateWithFlags(&e->htab_free, hipEventDisableTiming));
-    int rc = dev_alloc(e.get(), (void**)&e->tw14, 16384 * sizeof(float2));
+    int rc = slot_init(&e->batch, false);
+    if (rc != PA_OK) return rc;
+    rc = dev_alloc(e.get(), (void**)&e->tw14, 16384 * sizeof(float2));
     if (rc != PA_OK) return rc;
     auto tw = twiddles(16384, 16384, 1);
     PA_HIP(hipMemcpy(e->tw14, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice));
@@ -419,16 +458,9 @@ int pa_destroy(pa_engine* e) {
         (void)hipEventDestroy(p.b);
     }
     for (hipEvent_t ev : e->event_pool) (void)hipEventDestroy(ev);
-    for (auto& kv : e->graphs) (void)hipGraphExecDestroy(kv.second);
-    for (int i = 0; i < pa_engine::FORKS; ++i) {
-        (void)hipStreamSynchronize(e->fork[i]);
-        (void)hipStreamDestroy(e->fork[i]);
-        (void)hipEventDestroy(e->join_ev[i]);
-    }
-    (void)hipEventDestroy(e->fork_ev);
-    (void)hipEventDestroy(e->htab_free);
-    (void)hipFree(e->arena);
-    (void)hipHostFree(e->htab);
+    if (e->server) pa_serve_stop(e);
+    for (auto& kv : e->pinned) (void)hipHostFree(reinterpret_cast<void*>(kv.first));
+    slot_release(e, &e->batch);
     (void)hipFree(e->tw14);
     (void)hipStreamDestroy(e->stream);
     delete e;
@@ -630,6 +662,8 @@ int pa_wait(pa_engine* e, pa_ticket ticket, int64_t timeout_ns) {
     return PA_OK;
 }
 
+}  // extern "C"
+
 // ------------------------------------------------------------------ batched launches ----
 namespace {
 
@@ -711,18 +745,21 @@ int batch_plan(pa_engine* e, const pa_block* blocks, uint32_t count, BatchPlan* 
 
 // Grows the batch arena to at least `bytes` (drains the device first; cached graphs refer to the
 // old arena and are dropped).
-int ensure_arena(pa_engine* e, uint64_t bytes) {
-    if (bytes <= e->arena_cap) return PA_OK;
-    PA_HIP(hipDeviceSynchronize());
-    for (auto& kv : e->graphs) (void)hipGraphExecDestroy(kv.second);
-    e->graphs.clear();
-    if (e->arena) dev_free(e, e->arena, e->arena_cap);
-    e->arena = nullptr;
-    e->arena_cap = 0;
-    const uint64_t cap = align256(bytes + bytes / 8);
-    int rc = dev_alloc(e, (void**)&e->arena, cap);
+int ensure_arena(pa_engine* e, BatchSlot& sl, uint64_t bytes, uint64_t limit = 0) {
+    if (bytes <= sl.arena_cap) return PA_OK;
+    if (limit && bytes > limit) return PA_ERR_NOMEM;
+    if (sl.stream) PA_HIP(hipStreamSynchronize(sl.stream));
+    else PA_HIP(hipDeviceSynchronize());
+    for (auto& kv : sl.graphs) (void)hipGraphExecDestroy(kv.second);
+    sl.graphs.clear();
+    if (sl.arena) dev_free(e, sl.arena, sl.arena_cap);
+    sl.arena = nullptr;
+    sl.arena_cap = 0;
+    uint64_t cap = align256(bytes + bytes / 8);
+    if (limit && cap > limit) cap = align256(bytes);
+    int rc = dev_alloc(e, (void**)&sl.arena, cap);
     if (rc != PA_OK) return rc;
-    e->arena_cap = cap;
+    sl.arena_cap = cap;
     return PA_OK;
 }
 
@@ -751,31 +788,31 @@ int batch_group_launch(pa_engine* e, const BatchPlan& p, const BatchGroup& gr, c
 }
 
 // Groups on the fork streams (largest first onto the least-loaded stream), joined back into s.
-int batch_launch_all(pa_engine* e, const BatchPlan& p, const pa::BlockArgs* tab, hipStream_t s) {
+int batch_launch_all(pa_engine* e, BatchSlot& sl, const BatchPlan& p, const pa::BlockArgs* tab, hipStream_t s) {
     const size_t ng = p.groups.size();
     if (ng == 1) return batch_group_launch(e, p, p.groups[0], tab, p.groups[0].grid_x, s);
     std::vector<size_t> order(ng);
     for (size_t i = 0; i < ng; ++i) order[i] = i;
     std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return p.groups[a].cost > p.groups[b].cost; });
-    double load[pa_engine::FORKS] = {};
-    bool used[pa_engine::FORKS] = {};
-    PA_HIP(hipEventRecord(e->fork_ev, s));
+    double load[BatchSlot::FORKS] = {};
+    bool used[BatchSlot::FORKS] = {};
+    PA_HIP(hipEventRecord(sl.fork_ev, s));
     for (size_t gi : order) {
         int k = 0;
-        for (int j = 1; j < pa_engine::FORKS; ++j)
+        for (int j = 1; j < BatchSlot::FORKS; ++j)
             if (load[j] < load[k]) k = j;
         if (!used[k]) {
-            PA_HIP(hipStreamWaitEvent(e->fork[k], e->fork_ev, 0));
+            PA_HIP(hipStreamWaitEvent(sl.fork[k], sl.fork_ev, 0));
             used[k] = true;
         }
         load[k] += p.groups[gi].cost;
-        int rc = batch_group_launch(e, p, p.groups[gi], tab, p.groups[gi].grid_x, e->fork[k]);
+        int rc = batch_group_launch(e, p, p.groups[gi], tab, p.groups[gi].grid_x, sl.fork[k]);
         if (rc != PA_OK) return rc;
     }
-    for (int k = 0; k < pa_engine::FORKS; ++k) {
+    for (int k = 0; k < BatchSlot::FORKS; ++k) {
         if (!used[k]) continue;
-        PA_HIP(hipEventRecord(e->join_ev[k], e->fork[k]));
-        PA_HIP(hipStreamWaitEvent(s, e->join_ev[k], 0));
+        PA_HIP(hipEventRecord(sl.join_ev[k], sl.fork[k]));
+        PA_HIP(hipStreamWaitEvent(s, sl.join_ev[k], 0));
     }
     return PA_OK;
 }
@@ -820,20 +857,20 @@ int batch_tune_group(pa_engine* e, const BatchPlan& p, BatchGroup& gr, const pa:
 
 // Fills and uploads the block table, then launches (tuning / graph replay as asked).
 // `dev` holds the device pointers of key, seed, out and margin of every block.
-int batch_execute(pa_engine* e, BatchPlan& p, const pa_block* dev, uint32_t bflags, hipStream_t s) {
+int batch_execute(pa_engine* e, BatchSlot& sl, BatchPlan& p, const pa_block* dev, uint32_t bflags, hipStream_t s) {
     const uint32_t count = (uint32_t)p.items.size();
-    pa::BlockArgs* dtab = reinterpret_cast<pa::BlockArgs*>(e->arena);
-    uint8_t* scratch = e->arena + p.table_bytes;
-    if (e->htab_pending) {
-        PA_HIP(hipEventSynchronize(e->htab_free));
-        e->htab_pending = false;
+    pa::BlockArgs* dtab = reinterpret_cast<pa::BlockArgs*>(sl.arena);
+    uint8_t* scratch = sl.arena + p.table_bytes;
+    if (sl.htab_pending) {
+        PA_HIP(hipEventSynchronize(sl.htab_free));
+        sl.htab_pending = false;
     }
-    if (e->htab_cap < count) {
-        if (e->htab) (void)hipHostFree(e->htab);
-        e->htab = nullptr;
-        e->htab_cap = 0;
-        PA_HIP(hipHostMalloc((void**)&e->htab, (size_t)count * sizeof(pa::BlockArgs), hipHostMallocDefault));
-        e->htab_cap = count;
+    if (sl.htab_cap < count) {
+        if (sl.htab) (void)hipHostFree(sl.htab);
+        sl.htab = nullptr;
+        sl.htab_cap = 0;
+        PA_HIP(hipHostMalloc((void**)&sl.htab, (size_t)count * sizeof(pa::BlockArgs), hipHostMallocDefault));
+        sl.htab_cap = count;
     }
     uint64_t off = 0;
     for (auto& gr : p.groups) {
@@ -864,7 +901,7 @@ int batch_execute(pa_engine* e, BatchPlan& p, const pa_block* dev, uint32_t bfla
                 a.nib = take((uint64_t)g.L / 4 + 64);
                 a.counts = reinterpret_cast<uint32_t*>(take(2 * pa::POP_SLOTS * 4));
             }
-            e->htab[gr.tab0 + j] = a;
+            sl.htab[gr.tab0 + j] = a;
         }
         if (!gr.small) {
             auto t = e->tuned.find(tune_key(gr));
@@ -874,24 +911,24 @@ int batch_execute(pa_engine* e, BatchPlan& p, const pa_block* dev, uint32_t bfla
                                             : std::min(items, std::max<uint32_t>(1u, ((uint32_t)e->cus + cnt - 1u) / cnt));
         }
     }
-    PA_HIP(hipMemcpyAsync(dtab, e->htab, (size_t)count * sizeof(pa::BlockArgs), hipMemcpyHostToDevice, s));
-    PA_HIP(hipEventRecord(e->htab_free, s));
-    e->htab_pending = true;
+    PA_HIP(hipMemcpyAsync(dtab, sl.htab, (size_t)count * sizeof(pa::BlockArgs), hipMemcpyHostToDevice, s));
+    PA_HIP(hipEventRecord(sl.htab_free, s));
+    sl.htab_pending = true;
     if (bflags & PA_BATCH_TUNE)
         for (auto& gr : p.groups)
             if (!gr.small && e->tuned.find(tune_key(gr)) == e->tuned.end()) {
                 int rc = batch_tune_group(e, p, gr, dtab, s);
                 if (rc != PA_OK) return rc;
             }
-    if (!(bflags & PA_BATCH_GRAPH)) return batch_launch_all(e, p, dtab, s);
-    std::string key = std::to_string((uintptr_t)e->arena) + ":";
+    if (!(bflags & PA_BATCH_GRAPH)) return batch_launch_all(e, sl, p, dtab, s);
+    std::string key = std::to_string((uintptr_t)sl.arena) + ":";
     for (auto& gr : p.groups)
         key += tune_key(gr) + "x" + std::to_string(gr.grid_x) + "y" + std::to_string(gr.tiles_y) + ";";
-    auto f = e->graphs.find(key);
-    if (f == e->graphs.end()) {
+    auto f = sl.graphs.find(key);
+    if (f == sl.graphs.end()) {
         hipGraph_t graph = nullptr;
         PA_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        int rc = batch_launch_all(e, p, dtab, s);
+        int rc = batch_launch_all(e, sl, p, dtab, s);
         hipError_t ec = hipStreamEndCapture(s, &graph);
         if (rc != PA_OK) {
             if (graph) (void)hipGraphDestroy(graph);
@@ -902,7 +939,7 @@ int batch_execute(pa_engine* e, BatchPlan& p, const pa_block* dev, uint32_t bfla
         hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         PA_HIP(ei);
-        f = e->graphs.emplace(key, exec).first;
+        f = sl.graphs.emplace(key, exec).first;
     }
     PA_HIP(hipGraphLaunch(f->second, s));
     return PA_OK;
@@ -910,15 +947,17 @@ int batch_execute(pa_engine* e, BatchPlan& p, const pa_block* dev, uint32_t bfla
 
 }  // namespace
 
+extern "C" {
+
 int pa_batch_device(pa_engine* e, const pa_block* blocks, uint32_t count, uint32_t batch_flags, void* stream) {
     if (!e || (batch_flags & ~3u)) return PA_ERR_ARG;
     std::lock_guard<std::mutex> lk(e->mu);
     BatchPlan p;
     int rc = batch_plan(e, blocks, count, &p);
     if (rc != PA_OK) return rc;
-    rc = ensure_arena(e, p.table_bytes + p.scratch_bytes);
+    rc = ensure_arena(e, e->batch, p.table_bytes + p.scratch_bytes);
     if (rc != PA_OK) return rc;
-    return batch_execute(e, p, blocks, batch_flags, stream_of(e, stream));
+    return batch_execute(e, e->batch, p, blocks, batch_flags, stream_of(e, stream));
 }
 
 int pa_run_batch(pa_engine* e, const pa_block* blocks, uint32_t count, uint32_t batch_flags) {
@@ -943,26 +982,396 @@ int pa_run_batch(pa_engine* e, const pa_block* blocks, uint32_t count, uint32_t 
         ooff[i] = off;
         off += align256(out_bytes_of(p.items[i].g) + 16);
     }
-    rc = ensure_arena(e, off);
+    rc = ensure_arena(e, e->batch, off);
     if (rc != PA_OK) return rc;
     hipStream_t s = e->stream;
+    uint8_t* arena = e->batch.arena;
     std::vector<pa_block> dev(blocks, blocks + count);
-    PA_HIP(hipMemsetAsync(e->arena + key_lo, 0, key_hi - key_lo, s));
+    PA_HIP(hipMemsetAsync(arena + key_lo, 0, key_hi - key_lo, s));
     for (uint32_t i = 0; i < count; ++i) {
         const Geom& g = p.items[i].g;
-        dev[i].key = reinterpret_cast<const uint32_t*>(e->arena + koff[i]);
-        PA_HIP(hipMemcpyAsync(e->arena + koff[i], blocks[i].key, (size_t)key_words_for(g) * 4, hipMemcpyHostToDevice, s));
+        dev[i].key = reinterpret_cast<const uint32_t*>(arena + koff[i]);
+        PA_HIP(hipMemcpyAsync(arena + koff[i], blocks[i].key, (size_t)key_words_for(g) * 4, hipMemcpyHostToDevice, s));
         if (p.items[i].dyn) {
-            dev[i].seed = reinterpret_cast<const uint32_t*>(e->arena + soff[i]);
-            PA_HIP(hipMemcpyAsync(e->arena + soff[i], blocks[i].seed, (size_t)(g.n / 32) * 4, hipMemcpyHostToDevice, s));
+            dev[i].seed = reinterpret_cast<const uint32_t*>(arena + soff[i]);
+            PA_HIP(hipMemcpyAsync(arena + soff[i], blocks[i].seed, (size_t)(g.n / 32) * 4, hipMemcpyHostToDevice, s));
         }
-        dev[i].out = e->arena + ooff[i];
+        dev[i].out = arena + ooff[i];
     }
-    rc = batch_execute(e, p, dev.data(), batch_flags, s);
+    rc = batch_execute(e, e->batch, p, dev.data(), batch_flags, s);
     if (rc != PA_OK) return rc;
     for (uint32_t i = 0; i < count; ++i)
-        PA_HIP(hipMemcpyAsync(blocks[i].out, e->arena + ooff[i], out_bytes_of(p.items[i].g), hipMemcpyDeviceToHost, s));
+        PA_HIP(hipMemcpyAsync(blocks[i].out, arena + ooff[i], out_bytes_of(p.items[i].g), hipMemcpyDeviceToHost, s));
     PA_HIP(hipStreamSynchronize(s));
+    return PA_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ serving ----
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+// True if [p, p + bytes) lies inside one pa_host_alloc allocation.
+bool is_pinned(pa_engine* e, const void* p, uint64_t bytes) {
+    std::lock_guard<std::mutex> lk(e->pin_mu);
+    auto it = e->pinned.upper_bound(reinterpret_cast<uintptr_t>(p));
+    if (it == e->pinned.begin()) return false;
+    --it;
+    const uintptr_t a = reinterpret_cast<uintptr_t>(p);
+    return a >= it->first && a + bytes <= it->first + it->second;
+}
+
+struct ServeReq {
+    pa_block b{};                  // host pointers
+    bool pin_key = false, pin_seed = false, pin_out = false;  // DMA straight from / to the caller
+    int status = 1;                // 1 while pending, then the block's final status
+    Clock::time_point t_submit, t_done;
+    uint64_t dkey = 0, dseed = 0, dout = 0;  // staging offsets (device arena and pinned host)
+    size_t key_bytes = 0, seed_bytes = 0, out_bytes = 0;
+};
+
+struct ServeSlot {
+    BatchSlot bs;
+    bool busy = false;
+    hipEvent_t done = nullptr;
+    std::vector<ServeReq*> reqs;
+    uint8_t* hstage = nullptr;     // pinned host staging: inputs, then outputs
+    uint64_t hstage_cap = 0;
+    uint64_t bytes = 0;            // device bytes this batch holds against the budget
+    uint64_t in_lo = 0, in_hi = 0, out_lo = 0, out_hi = 0;  // staging ranges (arena offsets)
+};
+
+}  // namespace
+
+struct Server {
+    pa_serve_config cfg{};
+    std::thread th;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<ServeReq*> queue;
+    std::map<pa_ticket, ServeReq*> reqs;
+    std::vector<ServeSlot> slots;
+    uint64_t in_use = 0;
+    bool stop = false;
+    pa_ticket next = 1;
+    int init_status = PA_OK;
+};
+
+namespace {
+
+// Retires finished batches: outputs from pinned staging to the callers' buffers, then completion.
+bool serve_retire(pa_engine* e, Server* sv) {
+    bool any = false;
+    for (auto& sl : sv->slots) {
+        if (!sl.busy) continue;
+        const hipError_t q = hipEventQuery(sl.done);
+        if (q == hipErrorNotReady) continue;
+        const int st = q == hipSuccess ? PA_OK : PA_ERR_DEVICE;
+        if (q != hipSuccess) set_error(q, "serving batch", __FILE__, __LINE__);
+        for (ServeReq* r : sl.reqs)
+            if (st == PA_OK && !r->pin_out) memcpy(r->b.out, sl.hstage + (r->dout - sl.in_lo), r->out_bytes);
+        const auto now = Clock::now();
+        {
+            std::lock_guard<std::mutex> lk(sv->mu);
+            for (ServeReq* r : sl.reqs) {
+                r->status = st;
+                r->t_done = now;
+            }
+            sv->in_use -= sl.bytes;
+        }
+        sv->cv_done.notify_all();
+        sl.reqs.clear();
+        sl.busy = false;
+        sl.bytes = 0;
+        any = true;
+    }
+    return any;
+}
+
+void serve_fail(Server* sv, ServeReq* r, int st) {
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        r->status = st;
+        r->t_done = Clock::now();
+    }
+    sv->cv_done.notify_all();
+}
+
+// Device bytes a request holds while in flight: its scratch and its staging.
+uint64_t serve_req_bytes(const Geom& g, bool dyn) {
+    return batch_scratch_bytes(g, dyn) + align256((uint64_t)(g.n / 32 + 2) * 4) +
+           (dyn ? align256((uint64_t)(g.n / 32) * 4) : 0) + align256(out_bytes_of(g) + 16) +
+           align256(sizeof(pa::BlockArgs));
+}
+
+// Forms one batch from the queue head into a free slot and launches it.  Returns false when
+// nothing could be admitted.
+bool serve_admit(pa_engine* e, Server* sv, ServeSlot& sl) {
+    std::vector<ServeReq*> take;
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        while (!sv->queue.empty() && take.size() < sv->cfg.max_batch) {
+            take.push_back(sv->queue.front());
+            sv->queue.pop_front();
+        }
+    }
+    if (take.empty()) return false;
+    std::lock_guard<std::mutex> elk(e->mu);
+    // validate one by one; size against the budget; give back what does not fit
+    // every slot's arena stays within budget / slots, so all arenas together stay within budget
+    const uint64_t slot_limit = sv->cfg.hbm_budget_bytes / sv->slots.size();
+    std::vector<ServeReq*> ok;
+    std::vector<pa_block> blocks;
+    uint64_t bytes = 0;
+    const uint64_t budget_left = slot_limit - 4096;  // block table rounding
+    size_t i = 0;
+    for (; i < take.size(); ++i) {
+        ServeReq* r = take[i];
+        BatchPlan one;
+        int rc = batch_plan(e, &r->b, 1, &one);
+        if (rc != PA_OK) {
+            serve_fail(sv, r, rc);
+            continue;
+        }
+        const uint64_t rb = serve_req_bytes(one.items[0].g, one.items[0].dyn);
+        if (rb > budget_left) {
+            serve_fail(sv, r, PA_ERR_NOMEM);
+            continue;
+        }
+        if (bytes + rb > budget_left) break;
+        bytes += rb;
+        ok.push_back(r);
+        blocks.push_back(r->b);
+    }
+    if (i < take.size()) {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        for (size_t j = take.size(); j-- > i;) sv->queue.push_front(take[j]);
+    }
+    if (ok.empty()) return false;
+    BatchPlan p;
+    int rc = batch_plan(e, blocks.data(), (uint32_t)blocks.size(), &p);
+    auto fail_all = [&](int st) {
+        for (ServeReq* r : ok) serve_fail(sv, r, st);
+        return true;
+    };
+    if (rc != PA_OK) return fail_all(rc);
+    // staging after table + scratch: keys, seeds (one input range), then outputs
+    uint64_t off = p.table_bytes + p.scratch_bytes;
+    sl.in_lo = off;
+    for (size_t k = 0; k < ok.size(); ++k) {
+        ServeReq* r = ok[k];
+        const Geom& g = p.items[k].g;
+        r->key_bytes = (size_t)(g.n / 32 + 2) * 4;
+        r->dkey = off;
+        off += align256(r->key_bytes);
+        r->seed_bytes = p.items[k].dyn ? (size_t)(g.n / 32) * 4 : 0;
+        r->dseed = off;
+        off += align256(r->seed_bytes);
+    }
+    sl.in_hi = off;
+    sl.out_lo = off;
+    for (size_t k = 0; k < ok.size(); ++k) {
+        ok[k]->out_bytes = out_bytes_of(p.items[k].g);
+        ok[k]->dout = off;
+        off += align256(ok[k]->out_bytes + 16);
+    }
+    sl.out_hi = off;
+    rc = ensure_arena(e, sl.bs, off, slot_limit);
+    if (rc != PA_OK) return fail_all(rc);
+    const uint64_t hneed = sl.out_hi - sl.in_lo;
+    if (sl.hstage_cap < hneed) {
+        if (sl.hstage) (void)hipHostFree(sl.hstage);
+        sl.hstage = nullptr;
+        sl.hstage_cap = 0;
+        if (hipHostMalloc((void**)&sl.hstage, hneed, hipHostMallocDefault) != hipSuccess) return fail_all(PA_ERR_NOMEM);
+        sl.hstage_cap = hneed;
+    }
+    uint8_t* H = sl.hstage - sl.in_lo;  // H + arena offset = pinned address
+    uint8_t* A = sl.bs.arena;
+    std::vector<pa_block> dev(blocks);
+    std::vector<size_t> kws(ok.size());
+    for (size_t k = 0; k < ok.size(); ++k) {
+        ServeReq* r = ok[k];
+        const Geom& g = p.items[k].g;
+        kws[k] = (size_t)key_words_for(g) * 4;  // the kernels read no key word beyond these
+        r->pin_key = is_pinned(e, r->b.key, kws[k]);
+        r->pin_seed = r->seed_bytes && is_pinned(e, r->b.seed, r->seed_bytes);
+        r->pin_out = is_pinned(e, r->b.out, r->out_bytes);
+        if (!r->pin_key) memcpy(H + r->dkey, r->b.key, kws[k]);
+        if (r->seed_bytes && !r->pin_seed) memcpy(H + r->dseed, r->b.seed, r->seed_bytes);
+        dev[k].key = reinterpret_cast<const uint32_t*>(A + r->dkey);
+        if (r->seed_bytes) dev[k].seed = reinterpret_cast<const uint32_t*>(A + r->dseed);
+        dev[k].out = A + r->dout;
+        dev[k].margin = nullptr;
+    }
+    hipStream_t s = sl.bs.stream;
+    auto launch = [&]() -> int {
+        for (size_t k = 0; k < ok.size(); ++k) {
+            const ServeReq* r = ok[k];
+            const void* ks = r->pin_key ? (const void*)r->b.key : (const void*)(H + r->dkey);
+            PA_HIP(hipMemcpyAsync(A + r->dkey, ks, kws[k], hipMemcpyHostToDevice, s));
+            if (r->seed_bytes) {
+                const void* ss = r->pin_seed ? (const void*)r->b.seed : (const void*)(H + r->dseed);
+                PA_HIP(hipMemcpyAsync(A + r->dseed, ss, r->seed_bytes, hipMemcpyHostToDevice, s));
+            }
+        }
+        int r2 = batch_execute(e, sl.bs, p, dev.data(), sv->cfg.batch_flags, s);
+        if (r2 != PA_OK) return r2;
+        for (const ServeReq* r : ok) {
+            void* dst = r->pin_out ? (void*)r->b.out : (void*)(H + r->dout);
+            PA_HIP(hipMemcpyAsync(dst, A + r->dout, r->out_bytes, hipMemcpyDeviceToHost, s));
+        }
+        PA_HIP(hipEventRecord(sl.done, s));
+        return PA_OK;
+    };
+    rc = launch();
+    if (rc != PA_OK) return fail_all(rc);
+    sl.reqs = ok;
+    sl.bytes = bytes;
+    sl.busy = true;
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->in_use += bytes;
+    }
+    return true;
+}
+
+void serve_loop(pa_engine* e, Server* sv) {
+    (void)hipSetDevice(e->cfg.device);
+    while (true) {
+        bool progressed = serve_retire(e, sv);
+        for (auto& sl : sv->slots)
+            if (!sl.busy && serve_admit(e, sv, sl)) progressed = true;
+        bool busy = false;
+        for (auto& sl : sv->slots) busy = busy || sl.busy;
+        std::unique_lock<std::mutex> lk(sv->mu);
+        if (sv->stop && sv->queue.empty() && !busy) break;
+        if (progressed) continue;
+        if (busy) {
+            lk.unlock();
+            std::this_thread::sleep_for(std::chrono::microseconds(20));
+        } else {
+            sv->cv_work.wait_for(lk, std::chrono::milliseconds(2),
+                                 [&] { return sv->stop || !sv->queue.empty(); });
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int pa_host_alloc(pa_engine* e, uint64_t bytes, void** ptr) {
+    if (!e || !ptr || bytes == 0) return PA_ERR_ARG;
+    void* p = nullptr;
+    hipError_t r = hipHostMalloc(&p, bytes, hipHostMallocDefault);
+    if (r != hipSuccess) {
+        set_error(r, "hipHostMalloc", __FILE__, __LINE__);
+        return PA_ERR_NOMEM;
+    }
+    std::lock_guard<std::mutex> lk(e->pin_mu);
+    e->pinned[reinterpret_cast<uintptr_t>(p)] = bytes;
+    *ptr = p;
+    return PA_OK;
+}
+
+int pa_host_free(pa_engine* e, void* p) {
+    if (!e || !p) return PA_ERR_ARG;
+    {
+        std::lock_guard<std::mutex> lk(e->pin_mu);
+        auto it = e->pinned.find(reinterpret_cast<uintptr_t>(p));
+        if (it == e->pinned.end()) return PA_ERR_NOT_FOUND;
+        e->pinned.erase(it);
+    }
+    PA_HIP(hipHostFree(p));
+    return PA_OK;
+}
+
+int pa_serve_start(pa_engine* e, const pa_serve_config* cfg) {
+    if (!e) return PA_ERR_ARG;
+    if (e->server) return PA_ERR_ARG;
+    auto sv = std::make_unique<Server>();
+    sv->cfg.slots = 3;
+    sv->cfg.max_batch = 256;
+    sv->cfg.hbm_budget_bytes = 16ull << 30;
+    if (cfg) {
+        if (cfg->reserved || (cfg->batch_flags & ~3u)) return PA_ERR_ARG;
+        if (cfg->slots) sv->cfg.slots = cfg->slots;
+        if (cfg->max_batch) sv->cfg.max_batch = std::min<uint32_t>(cfg->max_batch, 65535u);
+        if (cfg->hbm_budget_bytes) sv->cfg.hbm_budget_bytes = cfg->hbm_budget_bytes;
+        sv->cfg.batch_flags = cfg->batch_flags;
+    }
+    if (sv->cfg.slots > 16) return PA_ERR_ARG;
+    sv->slots.resize(sv->cfg.slots);
+    for (auto& sl : sv->slots) {
+        int rc = slot_init(&sl.bs, true);
+        if (rc != PA_OK) return rc;
+        PA_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+    }
+    Server* raw = sv.release();
+    e->server = raw;
+    raw->th = std::thread(serve_loop, e, raw);
+    return PA_OK;
+}
+
+int pa_serve_submit(pa_engine* e, const pa_block* b, pa_ticket* ticket) {
+    if (!e || !e->server || !b || !ticket || !b->key || !b->out || (!b->seed && !b->seed_id) || b->margin)
+        return PA_ERR_ARG;
+    Server* sv = e->server;
+    auto* r = new ServeReq();
+    r->b = *b;
+    r->t_submit = Clock::now();
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        if (sv->stop) {
+            delete r;
+            return PA_ERR_ARG;
+        }
+        *ticket = sv->next++;
+        sv->reqs[*ticket] = r;
+        sv->queue.push_back(r);
+    }
+    sv->cv_work.notify_one();
+    return PA_OK;
+}
+
+int pa_serve_wait(pa_engine* e, pa_ticket ticket, int64_t timeout_ns, uint64_t* latency_ns) {
+    if (!e || !e->server) return PA_ERR_ARG;
+    Server* sv = e->server;
+    std::unique_lock<std::mutex> lk(sv->mu);
+    auto it = sv->reqs.find(ticket);
+    if (it == sv->reqs.end()) return PA_ERR_NOT_FOUND;
+    ServeReq* r = it->second;
+    auto ready = [&] { return r->status != 1; };
+    if (timeout_ns < 0) sv->cv_done.wait(lk, ready);
+    else if (!sv->cv_done.wait_for(lk, std::chrono::nanoseconds(timeout_ns), ready)) return PA_ERR_TIMEOUT;
+    const int st = r->status;
+    if (latency_ns)
+        *latency_ns = (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(r->t_done - r->t_submit).count();
+    sv->reqs.erase(it);
+    delete r;
+    return st;
+}
+
+int pa_serve_stop(pa_engine* e) {
+    if (!e || !e->server) return PA_ERR_ARG;
+    Server* sv = e->server;
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->stop = true;
+    }
+    sv->cv_work.notify_all();
+    if (sv->th.joinable()) sv->th.join();
+    for (auto& sl : sv->slots) {
+        slot_release(e, &sl.bs);
+        if (sl.done) (void)hipEventDestroy(sl.done);
+        if (sl.hstage) (void)hipHostFree(sl.hstage);
+    }
+    for (auto& kv : sv->reqs) delete kv.second;
+    delete sv;
+    e->server = nullptr;
     return PA_OK;
 }
 

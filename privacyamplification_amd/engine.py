@@ -12,6 +12,7 @@ import ctypes
 import os
 import pathlib
 import subprocess
+import weakref
 
 import numpy as np
 
@@ -68,6 +69,16 @@ BATCH_GRAPH = 1
 BATCH_TUNE = 2
 
 
+class PaServeConfig(ctypes.Structure):
+    _fields_ = [
+        ("slots", ctypes.c_uint32),
+        ("max_batch", ctypes.c_uint32),
+        ("hbm_budget_bytes", ctypes.c_uint64),
+        ("batch_flags", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
 def library_path():
     return _LIB
 
@@ -117,6 +128,12 @@ def load_library():
                                          ctypes.c_uint32, vp, vp, vp]
     L.pa_batch_device.argtypes = [eng, ctypes.POINTER(PaBlock), ctypes.c_uint32, ctypes.c_uint32, vp]
     L.pa_run_batch.argtypes = [eng, ctypes.POINTER(PaBlock), ctypes.c_uint32, ctypes.c_uint32]
+    L.pa_host_alloc.argtypes = [eng, ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]
+    L.pa_host_free.argtypes = [eng, ctypes.c_void_p]
+    L.pa_serve_start.argtypes = [eng, ctypes.POINTER(PaServeConfig)]
+    L.pa_serve_submit.argtypes = [eng, ctypes.POINTER(PaBlock), ctypes.POINTER(ctypes.c_uint64)]
+    L.pa_serve_wait.argtypes = [eng, ctypes.c_uint64, ctypes.c_int64, ctypes.POINTER(ctypes.c_uint64)]
+    L.pa_serve_stop.argtypes = [eng]
     L.pa_stage_timing.argtypes = [eng, ctypes.c_int]
     L.pa_stage_times.argtypes = [eng, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_double),
                                  ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]
@@ -129,7 +146,8 @@ def load_library():
     for name in ("pa_config_default", "pa_create", "pa_destroy", "pa_seed_put", "pa_seed_drop",
                  "pa_run_block", "pa_run_block_seeded", "pa_submit", "pa_wait", "pa_block_device",
                  "pa_block_device_seeded", "pa_synth_bits", "pa_stage_timing", "pa_stage_times",
-                 "pa_batch_device", "pa_run_batch"):
+                 "pa_batch_device", "pa_run_batch", "pa_serve_start", "pa_serve_submit",
+                 "pa_serve_wait", "pa_serve_stop", "pa_host_alloc", "pa_host_free"):
         getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L
@@ -175,9 +193,13 @@ class Engine:
         self._h = ctypes.c_void_p()
         _check(L.pa_create(ctypes.byref(cfg), ctypes.byref(self._h)), "pa_create")
         self._L = L
+        self._pinned = []  # finalizers of host_empty arrays (must not outlive the engine)
 
     def close(self):
         if self._h:
+            for f in getattr(self, "_pinned", []):  # pa_destroy frees every pinned buffer itself
+                f.detach()
+            self._pinned = []
             self._L.pa_destroy(self._h)
             self._h = ctypes.c_void_p()
 
@@ -281,6 +303,62 @@ class Engine:
         else:
             arr = blocks
         _check(self._L.pa_batch_device(self._h, arr, len(arr), batch_flags, stream), "pa_batch_device")
+
+    def host_empty(self, count, dtype=np.uint8):
+        """A numpy array in pinned host memory (pa_host_alloc); freed when the array is collected.
+        Blocks whose buffers are such arrays are copied by DMA without staging."""
+        dtype = np.dtype(dtype)
+        nbytes = max(1, int(count) * dtype.itemsize)
+        p = ctypes.c_void_p()
+        _check(self._L.pa_host_alloc(self._h, nbytes, ctypes.byref(p)), "pa_host_alloc")
+        buf = (ctypes.c_uint8 * nbytes).from_address(p.value)
+        arr = np.frombuffer(buf, dtype=dtype, count=int(count))
+        L, h, addr = self._L, self._h.value, p.value
+        self._pinned.append(weakref.finalize(buf, L.pa_host_free, h, addr))
+        return arr
+
+    def pinned_copy(self, a):
+        out = self.host_empty(a.size, a.dtype)
+        out[:] = a
+        return out
+
+    # ---- serving (continuous batching inside the engine)
+    def serve_start(self, slots=3, max_batch=256, hbm_budget_bytes=16 << 30, batch_flags=0):
+        cfg = PaServeConfig(slots, max_batch, hbm_budget_bytes, batch_flags, 0)
+        _check(self._L.pa_serve_start(self._h, ctypes.byref(cfg)), "pa_serve_start")
+        self._serve_keep = {}
+
+    def serve_submit(self, key, n, m, out, seed=None, sid=0, do_xor_key_rest=True, do_compress=True):
+        """Queues one block; `out` (uint8 array of at least out_bytes) receives amp_out.  The arrays
+        are kept alive until serve_wait returns for the ticket."""
+        b = PaBlock()
+        k, _ = _u32(key)
+        b.key = k.ctypes.data
+        sd = None
+        if seed is not None:
+            sd, _ = _u32(seed)
+            b.seed = sd.ctypes.data
+        b.seed_id, b.n, b.m = sid, n, m
+        b.flags = self.flags(do_xor_key_rest, do_compress)
+        assert out.dtype == np.uint8 and out.flags.c_contiguous and out.size >= out_bytes(n, m, do_compress)
+        b.out = out.ctypes.data
+        t = ctypes.c_uint64()
+        _check(self._L.pa_serve_submit(self._h, ctypes.byref(b), ctypes.byref(t)), "pa_serve_submit")
+        self._serve_keep[t.value] = (k, sd, out)
+        return t.value
+
+    def serve_wait(self, ticket, timeout_ns=-1):
+        """Returns the block's latency in nanoseconds (submit -> output written); raises on error."""
+        lat = ctypes.c_uint64()
+        st = self._L.pa_serve_wait(self._h, ticket, timeout_ns, ctypes.byref(lat))
+        if st != -5:
+            self._serve_keep.pop(ticket, None)
+        _check(st, "pa_serve_wait")
+        return lat.value
+
+    def serve_stop(self):
+        _check(self._L.pa_serve_stop(self._h), "pa_serve_stop")
+        self._serve_keep = {}
 
     def stage_timing(self, enable=True):
         _check(self._L.pa_stage_timing(self._h, 1 if enable else 0), "pa_stage_timing")

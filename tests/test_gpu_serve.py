@@ -1,0 +1,121 @@
+"""Serving (pa_serve_*): blocks submitted while earlier ones are on the GPU are admitted into
+batches as slots and HBM budget free up; every result must equal the block computed alone and the
+oracle.  Runs on an MI355X (marker gpu)."""
+import time
+
+import numpy as np
+import pytest
+
+import privacyamplification_amd as P
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+def _ref(oracle, key, seed, n, m, do_xor=True, do_compress=True):
+    return oracle.amp_out(key, seed, n, m, do_xor, do_compress, method=oracle.NTT)
+
+
+def test_serve_mixed_stream_equals_alone(engine, oracle):
+    rng = np.random.default_rng(0x5E4E)
+    shapes_static = [(1 << 14, 3 << 11), (96000, 24000)]
+    seeds = {}
+    for n, m in shapes_static:
+        s = oracle.synth_bits(0x5EED + n, n)
+        seeds[(n, m)] = (engine.seed_put(s, n, m), s)
+    engine.serve_start(slots=3, max_batch=32, hbm_budget_bytes=2 << 30)
+    try:
+        jobs = []
+        say("serve 80 mixed blocks")
+        for i in range(80):
+            if rng.random() < 0.6:
+                n, m = shapes_static[int(rng.integers(0, 2))]
+                sid, s = seeds[(n, m)]
+                key = oracle.synth_bits(0x7000 + i, n + 1)
+                out = np.zeros(m // 8 + 8, np.uint8)
+                t = engine.serve_submit(key, n, m, out, sid=sid)
+                jobs.append((t, out, key, s, n, m, True, True))
+            else:
+                e = rng.uniform(11, 20)
+                n = int(2 ** e) // 32 * 32
+                m = max(32, int(n * rng.choice([1, 2, 3, 4]) / 8) // 32 * 32)
+                pow2 = n & (n - 1) == 0
+                do_xor = bool(rng.random() < 0.7)
+                do_compress = True if not pow2 else bool(rng.random() < 0.8)
+                key = oracle.synth_bits(0x7000 + i, n + 1)
+                if not do_xor:
+                    key = key[: (n - m) // 32 + 1].copy()
+                s = oracle.synth_bits(0x8000 + i, n)
+                out = np.zeros(P.out_bytes(n, m, do_compress) + 8, np.uint8)
+                t = engine.serve_submit(key, n, m, out, seed=s, do_xor_key_rest=do_xor, do_compress=do_compress)
+                jobs.append((t, out, key, s, n, m, do_xor, do_compress))
+            if rng.random() < 0.3:
+                time.sleep(float(rng.uniform(0, 0.002)))  # arrivals while batches are in flight
+        lats = []
+        for t, out, key, s, n, m, do_xor, do_compress in jobs:
+            lats.append(engine.serve_wait(t, 60 * 10**9))
+            nb = P.out_bytes(n, m, do_compress)
+            alone = engine.amplify(key, s, n, m, do_xor, do_compress)
+            assert np.array_equal(out[:nb], alone), (n, m, do_xor, do_compress)
+            assert np.array_equal(out[:nb], _ref(oracle, key, s, n, m, do_xor, do_compress)), (n, m)
+        assert all(x > 0 for x in lats)
+    finally:
+        engine.serve_stop()
+        for sid, _ in seeds.values():
+            engine.seed_drop(sid)
+
+
+def test_serve_pinned_buffers(engine, oracle):
+    """Pinned (pa_host_alloc) keys, seeds and outputs go by DMA without staging; results unchanged,
+    also when pinned and pageable blocks share a batch."""
+    engine.serve_start(slots=2, max_batch=16)
+    try:
+        say("serve pinned + pageable")
+        jobs = []
+        for i in range(24):
+            n = [1 << 16, 3 << 14, 1 << 18][i % 3]
+            m = n // 4
+            key = oracle.synth_bits(0xA000 + i, n + 1)
+            seed = oracle.synth_bits(0xA100 + i, n)
+            if i % 2 == 0:
+                key, seed = engine.pinned_copy(key), engine.pinned_copy(seed)
+                out = engine.host_empty(m // 8)
+            else:
+                out = np.zeros(m // 8, np.uint8)
+            jobs.append((engine.serve_submit(key, n, m, out, seed=seed), out, key, seed, n, m))
+        for t, out, key, seed, n, m in jobs:
+            engine.serve_wait(t, 60 * 10**9)
+            assert np.array_equal(out, _ref(oracle, np.asarray(key), np.asarray(seed), n, m)), (n, m)
+    finally:
+        engine.serve_stop()
+
+
+def test_serve_budget_admission(engine, oracle):
+    """A small budget splits the stream into small batches; a block larger than a slot's share of
+    the budget is refused with PA_ERR_NOMEM and the others still complete."""
+    engine.serve_start(slots=2, max_batch=64, hbm_budget_bytes=64 << 20)
+    try:
+        n, m = 1 << 20, 3 << 17
+        say("serve budget 64 MiB, 12 blocks of 2^20 + one 2^24")
+        jobs = []
+        for i in range(12):
+            key = oracle.synth_bits(0x9000 + i, n + 1)
+            s = oracle.synth_bits(0x9100 + i, n)
+            out = np.zeros(m // 8 + 8, np.uint8)
+            jobs.append((engine.serve_submit(key, n, m, out, seed=s), out, key, s))
+        big_n = 1 << 24
+        big_key = oracle.synth_bits(1, big_n + 1)
+        big_seed = oracle.synth_bits(2, big_n)
+        big_out = np.zeros(big_n // 8, np.uint8)
+        tb = engine.serve_submit(big_key, big_n, 3 * big_n // 8, big_out, seed=big_seed)
+        for t, out, key, s in jobs:
+            engine.serve_wait(t, 60 * 10**9)
+            assert np.array_equal(out[: m // 8], _ref(oracle, key, s, n, m))
+        with pytest.raises(P.EngineError) as ei:
+            engine.serve_wait(tb, 60 * 10**9)
+        assert ei.value.status == -4
+    finally:
+        engine.serve_stop()

@@ -60,6 +60,26 @@ def test_conv_against_brute_force(oracle, n, kbits, nout, method):
     assert _bits_msb(got, nout) == _brute(ks, kbits, seed, n, nout)
 
 
+@pytest.mark.parametrize("n,kbits,nout", [(96, 40, 56), (160, 97, 63), (480, 200, 280), (992, 700, 292)])
+def test_ntt_any_length_against_brute_force(oracle, n, kbits, nout):
+    """Non-power-of-two n: the NTT's linear-convolution form equals brute force mod n."""
+    rng = np.random.default_rng(n * 7 + kbits)
+    ks = rng.integers(0, 2**32, n // 32, dtype=np.uint64).astype(np.uint32)
+    seed = rng.integers(0, 2**32, n // 32, dtype=np.uint64).astype(np.uint32)
+    got = oracle.conv_bits(ks, kbits, seed, n, nout, oracle.NTT)
+    assert _bits_msb(got, nout) == _brute(ks, kbits, seed, n, nout)
+
+
+@pytest.mark.parametrize("n,m", [(96000, 24000), (100000 - 100000 % 32, 25024), (5 << 12, 1 << 12)])
+def test_ntt_any_length_matches_direct(oracle, n, m):
+    key = oracle.synth_bits(n, n + 1)
+    seed = oracle.synth_bits(n + 1, n)
+    for do_xor in (True, False):
+        k = key if do_xor else key[: (n - m) // 32 + 1].copy()
+        assert np.array_equal(oracle.amp_out(k, seed, n, m, do_xor, True, method=oracle.DIRECT),
+                              oracle.amp_out(k, seed, n, m, do_xor, True, method=oracle.NTT))
+
+
 @pytest.mark.parametrize("e", [10, 12, 16, 20])
 def test_methods_agree(oracle, e):
     n = 1 << e
