@@ -274,6 +274,73 @@ def bench_serving(args):
     eng.close()
 
 
+def bench_precision(args):
+    """Precision sweep: for n in {2^24, 2^26, 2^28, 2^29}, key Bernoulli(0.5) and (0.9), seed
+    Bernoulli(0.5), m = 3n/8: worst |v - rint(v)| before rounding and SHA3 equality with the exact
+    digest, for the full-length FP32 transform and for FP32 sub-blocks b in {2^20, 2^22, 2^24}
+    (b < n); the FP64 and reference-FP32 margins come from the CPU fixture precision_cpu.json."""
+    import numpy as np
+    import torch
+    import privacyamplification_amd as P
+
+    dev = torch.device("cuda:0")
+    man = [b for b in json.loads((ROOT / "tests/golden/manifest.json").read_text())["blocks"]
+           if b["name"].startswith("c3_")]
+    cpu_file = ROOT / "tests/golden/precision_cpu.json"
+    cpu = json.loads(cpu_file.read_text())["blocks"] if cpu_file.exists() else {}
+    eng = P.Engine(device=0, max_n_log2=29)
+    stream = torch.cuda.Stream(device=dev)
+    torch.cuda.set_stream(stream)
+    sptr = stream.cuda_stream
+    rows = []
+    for b in man:
+        n, m = b["n"], b["m"]
+        key = P.synth_bits(b["key_seed"], n + 1, b["p_key"])
+        seed = P.synth_bits(b["toeplitz_seed"], n)
+        dk = torch.from_numpy(np.concatenate([key, np.zeros(1, np.uint32)]).view(np.int32)).to(dev)
+        ds = torch.from_numpy(seed.view(np.int32)).to(dev)
+        out = torch.zeros(m // 8 + 16, dtype=torch.uint8, device=dev)
+        mg = torch.zeros(1, dtype=torch.float32, device=dev)
+
+        def run(fn):
+            out.zero_()
+            mg.zero_()
+            torch.cuda.synchronize()
+            a, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            fn()
+            e.record(stream)
+            torch.cuda.synchronize()
+            ok = hashlib.sha3_256(out[: m // 8].cpu().numpy().tobytes()).hexdigest() == b["sha3"]
+            return {"margin": round(float(mg.item()), 6), "exact": ok, "ms": round(a.elapsed_time(e), 3)}
+
+        row = {"name": b["name"], "n": n, "p_key": b["p_key"]}
+        row["fp32_full"] = run(lambda: eng.block_device(dk.data_ptr(), ds.data_ptr(), n, m, P.DEFAULT_FLAGS,
+                                                        out.data_ptr(), sptr, mg.data_ptr()))
+        for sub in (20, 22, 24):
+            if (1 << sub) >= n:
+                continue
+            row[f"fp32_sub_2^{sub}"] = run(lambda: eng.block_device_split(
+                dk.data_ptr(), ds.data_ptr(), n, m, P.DEFAULT_FLAGS, sub, out.data_ptr(), sptr, mg.data_ptr()))
+        c = cpu.get(b["name"], {})
+        for k in ("fp64_margin", "fp64_exact", "ref_fp32_margin", "ref_fp32_exact"):
+            row[k] = c.get(k)
+        rows.append(row)
+        print(json.dumps({"progress": row}), file=sys.stderr, flush=True)
+        del dk, ds, out
+    worst = max(r["fp32_full"]["margin"] for r in rows)
+    print(json.dumps({
+        "metric": "precision sweep: worst |v - rint(v)| before rounding, FP32 full length (n = 2^24..2^29)",
+        "value": worst, "unit": "distance to nearest integer (fails at 0.5)", "n_gpus": 1, "steps": len(rows),
+        "warmup": 0, "ms_per_step": None, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "n in {2^24, 2^26, 2^28, 2^29}, key p in {0.5, 0.9}, seed p 0.5, m = 3n/8, "
+                               "XOR + compress; FP32 full length and FP32 sub-blocks b in {2^20, 2^22, 2^24}"},
+        "all_exact": all(v["exact"] for r in rows for k, v in r.items() if k.startswith("fp32_")),
+        "rows": rows}), flush=True)
+    eng.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -282,10 +349,11 @@ def main():
     ap.add_argument("--log2n", type=int, default=27)
     ap.add_argument("--mode", choices=("dynamic", "static"), default="dynamic")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=("block", "batched", "serving"), default="block",
+    ap.add_argument("--workload", choices=("block", "batched", "serving", "precision"), default="block",
                     help="block: one 2^27-bit block per step (headline); batched: 256 uneven blocks "
                          "per step through pa_batch_device vs one launch sequence per block; "
-                         "serving: Poisson arrivals at 2/8/32 Gbit/s through pa_serve_*")
+                         "serving: Poisson arrivals at 2/8/32 Gbit/s through pa_serve_*; precision: "
+                         "rounding margins at n = 2^24..2^29, full-length FP32 and FP32 sub-blocks")
     ap.add_argument("--serve-blocks", type=int, default=10000, help="blocks per offered rate (serving)")
     ap.add_argument("--rates", default="2,8,32", help="offered loads in Gbit/s (serving)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -305,6 +373,8 @@ def main():
         return bench_batched(args)
     if args.workload == "serving":
         return bench_serving(args)
+    if args.workload == "precision":
+        return bench_precision(args)
     dev = torch.device("cuda:0")
     n = 1 << args.log2n
     m = 3 * n // 8

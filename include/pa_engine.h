@@ -141,6 +141,21 @@ int pa_batch_device(pa_engine* eng, const pa_block* blocks, uint32_t count, uint
 /* Same with host buffers; returns when every output has been written. */
 int pa_run_batch(pa_engine* eng, const pa_block* blocks, uint32_t count, uint32_t batch_flags);
 
+/* Sub-block (tiled) evaluation of one block, the device form of LargeBlocksizeExample's tiling
+ * (LBS/main.cpp:104-124, 313-523) used as the precision fallback: the m x (h+1) Toeplitz product is
+ * cut into b x b tiles, b = 2^sub_log2; each tile is a length-2b circular product through the
+ * batched engine (the LBS tile contract: NoXOR, no compression), tiles XOR into their output band,
+ * then the key rest is XORed.  Every convolution value stays below b, so FP32 keeps a wide rounding
+ * margin at any n; the cost grows as n^2 / b.  Output identical to pa_block_device.  Requires
+ * PA_COMPRESS and 2^6 <= b < n, 2b <= 2^max_n_log2; n, m multiples of 32 (n < 2^31).
+ * d_margin as in pa_block_device (max over all tiles). */
+int pa_block_device_split(pa_engine* eng, const uint32_t* d_key, const uint32_t* d_seed, uint32_t n,
+                          uint32_t m, uint32_t flags, uint32_t sub_log2, uint8_t* d_out, void* stream,
+                          float* d_margin);
+/* Same with host buffers (synchronous). */
+int pa_run_block_split(pa_engine* eng, const uint32_t* key, const uint32_t* seed, uint32_t n,
+                       uint32_t m, uint32_t flags, uint32_t sub_log2, uint8_t* out);
+
 /* Pinned host memory for key, seed and output buffers: the counterpart of the reference's
  * cudaMallocHost'd input and output block caches (PrivacyAmplification.cu:1535-1551).  Blocks
  * whose buffers lie in such memory are copied by DMA straight from and to it; other host memory

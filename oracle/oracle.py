@@ -44,6 +44,7 @@ def lib():
         L.pa_or_element_wise_product.argtypes = [f32p, f32p, ctypes.c_uint32, ctypes.c_float]
         L.pa_or_to_binary_array.argtypes = [f32p, ctypes.c_uint32, ctypes.c_float, ctypes.c_float,
                                             ctypes.c_void_p, ctypes.c_int, u32p]
+        L.pa_or_set_ntt_big.argtypes = [ctypes.c_int]
         L.pa_or_conv_bits.argtypes = [u32p, ctypes.c_uint32, u32p, ctypes.c_uint32, ctypes.c_uint32,
                                       u8p, ctypes.c_int]
         L.pa_or_conv_bits.restype = ctypes.c_int
@@ -151,3 +152,8 @@ def amp_out(key, seed, n, m, do_xor=True, do_compress=True, reverse_endian=True,
     if rc < 0:
         raise ValueError("pa_or_amp_out: bad arguments")
     return out[: (out_bits + 7) // 8] if reverse_endian else out[: ((out_bits + 31) // 32) * 4]
+
+
+def set_ntt_big(on):
+    """Test hook: route every NTT through the long-length prime 3*2^30+1."""
+    lib().pa_or_set_ntt_big(1 if on else 0)

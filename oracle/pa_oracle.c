@@ -298,32 +298,114 @@ static int ntt_cyclic(uint32_t* a, uint32_t* b, uint32_t Lc) {
     return 0;
 }
 
+/* Lengths above 2^27: the prime 3*2^30+1 (primitive root 5) has roots of order up to 2^30.  It
+ * exceeds 2^31, so this NTT works in plain 64-bit arithmetic: twiddle products by Shoup's method
+ * (precomputed floor(w 2^32 / p)), the pointwise product by a 64-bit remainder. */
+#define NTT2_P 3221225473u
+static int ntt_force_big = 0;  /* test hook: use the 64-bit prime at every length */
+void pa_or_set_ntt_big(int on) { ntt_force_big = on; }
+static inline uint32_t p2_add(uint32_t a, uint32_t b) { uint64_t s = (uint64_t)a + b; return (uint32_t)(s >= NTT2_P ? s - NTT2_P : s); }
+static inline uint32_t p2_sub(uint32_t a, uint32_t b) { return a >= b ? a - b : (uint32_t)((uint64_t)a + NTT2_P - b); }
+static inline uint32_t p2_mul(uint32_t a, uint32_t b) { return (uint32_t)(((uint64_t)a * b) % NTT2_P); }
+static inline uint32_t p2_mul_shoup(uint32_t a, uint32_t w, uint32_t wq) {
+    uint64_t q = ((uint64_t)a * wq) >> 32;
+    uint64_t r = (uint64_t)a * w - q * NTT2_P;
+    return (uint32_t)(r >= NTT2_P ? r - NTT2_P : r);
+}
+static uint32_t p2_pow(uint32_t b, uint64_t e) {
+    uint32_t r = 1;
+    while (e) { if (e & 1) r = p2_mul(r, b); b = p2_mul(b, b); e >>= 1; }
+    return r;
+}
+static void p2_tables(uint32_t w, uint32_t half, uint32_t* tw, uint32_t* twq) {
+    uint32_t chunk = 4096;
+#pragma omp parallel for schedule(static)
+    for (int64_t c0 = 0; c0 < (int64_t)half; c0 += chunk) {
+        uint32_t x = p2_pow(w, (uint64_t)c0);
+        for (uint32_t j = (uint32_t)c0; j < half && j < (uint32_t)c0 + chunk; ++j) {
+            tw[j] = x;
+            twq[j] = (uint32_t)(((uint64_t)x << 32) / NTT2_P);
+            x = p2_mul(x, w);
+        }
+    }
+}
+static void p2_core(uint32_t* a, uint32_t n, const uint32_t* tw, const uint32_t* twq, int inverse) {
+    if (!inverse) {
+        for (uint32_t len = n; len >= 2; len >>= 1) {
+            uint32_t half = len / 2, stride = n / len;
+#pragma omp parallel for schedule(static)
+            for (int64_t blk = 0; blk < (int64_t)(n / len); ++blk) {
+                uint32_t* x = a + (uint64_t)blk * len;
+                for (uint32_t j = 0; j < half; ++j) {
+                    uint32_t u = x[j], v = x[j + half];
+                    x[j] = p2_add(u, v);
+                    x[j + half] = p2_mul_shoup(p2_sub(u, v), tw[(uint64_t)j * stride], twq[(uint64_t)j * stride]);
+                }
+            }
+        }
+    } else {
+        for (uint32_t len = 2; len <= n; len <<= 1) {
+            uint32_t half = len / 2, stride = n / len;
+#pragma omp parallel for schedule(static)
+            for (int64_t blk = 0; blk < (int64_t)(n / len); ++blk) {
+                uint32_t* x = a + (uint64_t)blk * len;
+                for (uint32_t j = 0; j < half; ++j) {
+                    uint32_t u = x[j], v = p2_mul_shoup(x[j + half], tw[(uint64_t)j * stride], twq[(uint64_t)j * stride]);
+                    x[j] = p2_add(u, v);
+                    x[j + half] = p2_sub(u, v);
+                }
+            }
+        }
+    }
+}
+/* a <- a (*) b cyclically over Lc <= 2^30, plain residues (0/1 inputs) in and out. */
+static int ntt2_cyclic(uint32_t* a, uint32_t* b, uint32_t Lc) {
+    uint32_t half = Lc / 2;
+    uint32_t* tw = (uint32_t*)malloc((size_t)half * 4 + 4);
+    uint32_t* twq = (uint32_t*)malloc((size_t)half * 4 + 4);
+    if (!tw || !twq) { free(tw); free(twq); return -1; }
+    uint32_t w = p2_pow(5, (uint64_t)(NTT2_P - 1) / Lc);
+    p2_tables(w, half, tw, twq);
+    p2_core(a, Lc, tw, twq, 0);
+    p2_core(b, Lc, tw, twq, 0);
+#pragma omp parallel for schedule(static)
+    for (int64_t j = 0; j < (int64_t)Lc; ++j) a[j] = p2_mul(a[j], b[j]);
+    p2_tables(p2_pow(w, (uint64_t)Lc - 1), half, tw, twq);
+    p2_core(a, Lc, tw, twq, 1);
+    uint32_t ninv = p2_pow(Lc, (uint64_t)NTT2_P - 2);
+#pragma omp parallel for schedule(static)
+    for (int64_t j = 0; j < (int64_t)Lc; ++j) a[j] = p2_mul(a[j], ninv);
+    free(tw); free(twq);
+    return 0;
+}
+
 /* counts[p] = sum_{j < kbits} x[j] s[(p - j) mod n] exactly, p < nout.  Power-of-two n: one cyclic
  * NTT of length n.  Other n: the linear convolution of x with the seed repeated twice (length 2n)
  * read at p + n, on an NTT of length >= kbits + 2n so nothing wraps.  Exact while every sum stays
- * below the prime (kbits < 2^31) and the length fits its 2^27-th roots. */
+ * below the prime (kbits < 2^31); lengths up to 2^27 use 15*2^27+1, longer ones 3*2^30+1. */
 int pa_or_conv_counts(const uint32_t* ks, uint32_t kbits, const uint32_t* seed, uint32_t n,
                       uint32_t nout, uint32_t* counts) {
     if (n < 2 || kbits > n || nout > n) return -1;
     const int pow2 = (n & (n - 1)) == 0;
     uint64_t Lc = 1;
     while (Lc < (pow2 ? (uint64_t)n : (uint64_t)kbits + 2ull * n)) Lc <<= 1;
-    if (Lc > (1ull << 27)) return -1;
+    if (Lc > (1ull << 30)) return -1;
+    const int big = Lc > (1ull << 27) || ntt_force_big;  /* beyond 15*2^27+1's roots: 3*2^30+1 */
     ntt_init();
     uint32_t* a = (uint32_t*)calloc((size_t)Lc, 4);
     uint32_t* b = (uint32_t*)calloc((size_t)Lc, 4);
     if (!a || !b) { free(a); free(b); return -1; }
-    uint32_t one = to_mont(1);
+    uint32_t one = big ? 1u : to_mont(1);
     const uint64_t bl = pow2 ? n : 2ull * n;
 #pragma omp parallel for schedule(static)
     for (int64_t j = 0; j < (int64_t)kbits; ++j) a[j] = refbit(ks, (uint64_t)j) ? one : 0u;
 #pragma omp parallel for schedule(static)
     for (int64_t t = 0; t < (int64_t)bl; ++t) b[t] = refbit(seed, (uint64_t)t % n) ? one : 0u;
-    int rc = ntt_cyclic(a, b, (uint32_t)Lc);
+    int rc = big ? ntt2_cyclic(a, b, (uint32_t)Lc) : ntt_cyclic(a, b, (uint32_t)Lc);
     if (rc == 0) {
         const uint64_t base = pow2 ? 0 : n;
 #pragma omp parallel for schedule(static)
-        for (int64_t p = 0; p < (int64_t)nout; ++p) counts[p] = from_mont(a[base + (uint64_t)p]);
+        for (int64_t p = 0; p < (int64_t)nout; ++p) counts[p] = big ? a[base + (uint64_t)p] : from_mont(a[base + (uint64_t)p]);
     }
     free(a); free(b);
     return rc;

@@ -51,8 +51,9 @@ void pa_or_to_binary_array(const float* v, uint32_t nbits, float norm, float cor
 /* Circular GF(2) product: conv[p] = XOR_{j<kbits} x[j] & s[(p-j) mod n] for p < nout,
  * x = first kbits bits of key_start (RefBits), s = n-bit seed (RefBits).
  * Written as an MSB-first bit string (OutBytes layout) of ceil(nout/8) bytes.
- * method: PA_OR_DIRECT (bit-packed schoolbook), PA_OR_NTT (exact integer NTT mod 15*2^27+1,
- * n a power of two <= 2^27), PA_OR_FFT64 (double FFT, rounded).  Returns 0 on success. */
+ * method: PA_OR_DIRECT (bit-packed schoolbook), PA_OR_NTT (exact integer NTT mod 15*2^27+1, or
+ * mod 3*2^30+1 for transform lengths above 2^27; any n with n <= 2^29 when n is a power of two),
+ * PA_OR_FFT64 (double FFT, rounded).  Returns 0 on success. */
 int pa_or_conv_bits(const uint32_t* key_start, uint32_t kbits, const uint32_t* seed, uint32_t n,
                     uint32_t nout, uint8_t* out_bytes, int method);
 
@@ -73,9 +74,12 @@ int pa_or_fft_values(const uint32_t* key_start, uint32_t kbits, const uint32_t* 
                      uint32_t nout, int use_double, double* values);
 
 /* Exact integer circular convolution c[p] = sum_{j<kbits} x[j] s[(p-j) mod n] for p < nout
- * (n a power of two <= 2^27). */
+ * (n a power of two <= 2^30, or any n with kbits + 2n <= 2^30). */
 int pa_or_conv_counts(const uint32_t* key_start, uint32_t kbits, const uint32_t* seed, uint32_t n,
                       uint32_t nout, uint32_t* counts);
+
+/* Test hook: 1 makes every NTT use the prime 3*2^30+1 (the long-length path), 0 restores. */
+void pa_or_set_ntt_big(int on);
 
 /* Threads used by the OpenMP regions (0 = library default). */
 void pa_or_set_threads(int nthreads);

@@ -82,7 +82,7 @@ int make_geom(uint32_t n, uint32_t m, uint32_t flags, uint32_t reverse_endian, u
         x.N2 = 1u << x.lg2;
         x.lg1 = x.lgN - x.lg2;
         x.N1 = 1u << x.lg1;
-        x.M = x.N1 < 512 ? x.N1 : 512;
+        x.M = x.N1 <= 512 ? x.N1 : (x.N1 / 16 > 512 ? x.N1 / 16 : 512);  // D <= 16 bit classes
         x.D = x.N1 / x.M;
         x.W = (uint32_t)pa::inv_cols_width((int)x.lg1);
     }
@@ -181,6 +181,8 @@ struct pa_engine {
     std::map<std::string, uint32_t> tuned;   // row-pass grid per (L, seed mode, count)
     struct Server* server = nullptr;         // pa_serve_*
     std::map<uintptr_t, uint64_t> pinned;    // pa_host_alloc ranges: base -> bytes
+    uint8_t* split = nullptr;                // sub-block mode: tile inputs/outputs, band accumulator,
+    uint64_t split_cap = 0;                  // host-API staging (grow-only)
     std::mutex pin_mu;
 };
 
@@ -249,7 +251,7 @@ uint64_t workspace_bytes(const Geom& g) {
     uint64_t b = (16384 + (N > 8192 ? g.L / 16384 : 1)) * sizeof(float2);
     if (g.N1 > 1) {
         b += N * sizeof(float2) + (N + 1) * sizeof(float2);
-        b += 2 * (N / g.D) * 4 + g.L / 4 + 256;
+        b += 2 * (N / g.D) * 4 + pa::nib_bytes(g.N, g.W) + 192;
     }
     b += 256 + (g.n / 32 + 2) * 4 * 2 + (g.n / 16 + 4);
     return b;
@@ -302,7 +304,7 @@ int get_workspace(pa_engine* e, const Geom& g, Workspace** out) {
         allocs.push_back({(void**)&w->Ys, (size_t)(g.N + 1) * sizeof(float2)});
         allocs.push_back({(void**)&w->tbk, (size_t)(g.N / g.D) * 4});
         allocs.push_back({(void**)&w->tbs, (size_t)(g.N / g.D) * 4});
-        allocs.push_back({(void**)&w->nib, (size_t)g.L / 4 + 64});
+        allocs.push_back({(void**)&w->nib, (size_t)pa::nib_bytes(g.N, g.W)});
     }
     for (auto& a : allocs) {
         rc = dev_alloc(e, a.p, a.b);
@@ -461,6 +463,7 @@ int pa_destroy(pa_engine* e) {
     if (e->server) pa_serve_stop(e);
     for (auto& kv : e->pinned) (void)hipHostFree(reinterpret_cast<void*>(kv.first));
     slot_release(e, &e->batch);
+    if (e->split) (void)hipFree(e->split);
     (void)hipFree(e->tw14);
     (void)hipStreamDestroy(e->stream);
     delete e;
@@ -696,7 +699,7 @@ uint64_t batch_scratch_bytes(const Geom& g, bool dyn) {
     if (g.N1 == 1) return 0;
     uint64_t b = align256((uint64_t)g.N * 8) + align256((uint64_t)(g.N / g.D) * 4);  // Yk, tbk
     if (dyn) b += align256((uint64_t)(g.N + 1) * 8) + align256((uint64_t)(g.N / g.D) * 4);  // Ys, tbs
-    b += align256((uint64_t)g.L / 4 + 64) + align256(2 * pa::POP_SLOTS * 4);  // nib, counts
+    b += align256(pa::nib_bytes(g.N, g.W)) + align256(2 * pa::POP_SLOTS * 4);  // nib, counts
     return b;
 }
 
@@ -898,7 +901,7 @@ int batch_execute(pa_engine* e, BatchSlot& sl, BatchPlan& p, const pa_block* dev
                     a.Ys = reinterpret_cast<float2*>(take((uint64_t)(g.N + 1) * 8));
                     a.tbs = reinterpret_cast<uint32_t*>(take((uint64_t)(g.N / g.D) * 4));
                 }
-                a.nib = take((uint64_t)g.L / 4 + 64);
+                a.nib = take(pa::nib_bytes(g.N, g.W));
                 a.counts = reinterpret_cast<uint32_t*>(take(2 * pa::POP_SLOTS * 4));
             }
             sl.htab[gr.tab0 + j] = a;
@@ -1002,6 +1005,112 @@ int pa_run_batch(pa_engine* e, const pa_block* blocks, uint32_t count, uint32_t 
     if (rc != PA_OK) return rc;
     for (uint32_t i = 0; i < count; ++i)
         PA_HIP(hipMemcpyAsync(blocks[i].out, arena + ooff[i], out_bytes_of(p.items[i].g), hipMemcpyDeviceToHost, s));
+    PA_HIP(hipStreamSynchronize(s));
+    return PA_OK;
+}
+
+static int split_run(pa_engine* e, const uint32_t* d_key, const uint32_t* d_seed, uint32_t n, uint32_t m,
+                     uint32_t flags, uint32_t sub_log2, uint32_t* d_out, hipStream_t s, float* d_margin,
+                     uint64_t extra_bytes, uint8_t** extra) {
+    if (flags & ~3u) return PA_ERR_ARG;
+    if (!(flags & PA_COMPRESS)) return PA_ERR_UNSUPPORTED;
+    if (n < 64 || (n % 32) || m == 0 || (m % 32) || m >= n || n >= 0x80000000u) return PA_ERR_SIZE;
+    if (sub_log2 < 6 || sub_log2 + 1 > e->cfg.max_n_log2 || sub_log2 + 1 > (uint32_t)pa::MAX_L_LOG) return PA_ERR_SIZE;
+    const uint32_t b = 1u << sub_log2;
+    if (b >= n) return PA_ERR_SIZE;
+    const uint32_t h = n - m, wb = b / 32;
+    const uint32_t nbands = (m + b - 1) / b, ncols = (h + 1 + b - 1) / b;
+    // one tile's footprint: inputs, output, batch scratch and table row
+    Geom tg;
+    int rc = make_geom(2 * b, b, 0, e->cfg.out_reverse_endian, e->cfg.max_n_log2, &tg);
+    if (rc != PA_OK) return rc;
+    const uint64_t tio = align256((uint64_t)(wb + 1) * 4) + align256((uint64_t)2 * wb * 4) + align256((uint64_t)wb * 4);
+    const uint64_t tile_all = tio + batch_scratch_bytes(tg, true) + sizeof(pa::BlockArgs);
+    const uint64_t chunk_budget = std::max<uint64_t>(tile_all * nbands, std::min<uint64_t>(8ull << 30, e->cfg.hbm_budget_bytes / 4));
+    uint32_t cols = (uint32_t)std::max<uint64_t>(1, chunk_budget / (tile_all * nbands));
+    cols = std::min(cols, std::max(1u, 65535u / nbands));
+    cols = std::min(cols, ncols);
+    if ((uint64_t)cols * nbands > 65535u) return PA_ERR_SIZE;
+    const uint32_t tiles = cols * nbands;
+    // split arena: tile keys | tile seeds | tile outputs | band accumulator | caller's extra
+    const uint64_t kb = align256((uint64_t)tiles * (wb + 1) * 4), sb = align256((uint64_t)tiles * 2 * wb * 4);
+    const uint64_t ob = align256((uint64_t)tiles * wb * 4), ab = align256((uint64_t)nbands * wb * 4 + 16);
+    const uint64_t need = kb + sb + ob + ab + align256(extra_bytes);
+    if (need > e->split_cap) {
+        PA_HIP(hipStreamSynchronize(s));
+        if (e->split) dev_free(e, e->split, e->split_cap);
+        e->split = nullptr;
+        e->split_cap = 0;
+        rc = dev_alloc(e, (void**)&e->split, need);
+        if (rc != PA_OK) return rc;
+        e->split_cap = need;
+    }
+    uint32_t* tkeys = reinterpret_cast<uint32_t*>(e->split);
+    uint32_t* tseeds = reinterpret_cast<uint32_t*>(e->split + kb);
+    uint32_t* touts = reinterpret_cast<uint32_t*>(e->split + kb + sb);
+    uint32_t* acc = reinterpret_cast<uint32_t*>(e->split + kb + sb + ob);
+    if (extra) {
+        *extra = e->split + kb + sb + ob + ab;
+        return PA_OK;  // the caller stages its buffers, then calls again with extra == nullptr
+    }
+    std::vector<pa_block> blocks(tiles);
+    for (uint32_t t = 0; t < tiles; ++t) {
+        pa_block& bl = blocks[t];
+        bl = pa_block{};
+        bl.key = tkeys + (size_t)t * (wb + 1);
+        bl.seed = tseeds + (size_t)t * 2 * wb;
+        bl.n = 2 * b;
+        bl.m = b;
+        bl.flags = 0;
+        bl.out = reinterpret_cast<uint8_t*>(touts + (size_t)t * wb);
+        bl.margin = d_margin;
+    }
+    for (uint32_t j0 = 0; j0 < ncols; j0 += cols) {
+        const uint32_t cc = std::min(cols, ncols - j0);
+        PA_HIP(pa::launch_tile_inputs(d_key, d_seed, n, h, b, nbands, j0, cc * nbands, tkeys, tseeds, s));
+        BatchPlan p;
+        rc = batch_plan(e, blocks.data(), cc * nbands, &p);
+        if (rc != PA_OK) return rc;
+        rc = ensure_arena(e, e->batch, p.table_bytes + p.scratch_bytes);
+        if (rc != PA_OK) return rc;
+        rc = batch_execute(e, e->batch, p, blocks.data(), PA_BATCH_GRAPH, s);
+        if (rc != PA_OK) return rc;
+        PA_HIP(pa::launch_tile_xor(acc, touts, nbands, wb, cc, j0 == 0, s));
+    }
+    PA_HIP(pa::launch_split_finish(acc, d_key, d_out, m, h, (flags & PA_XOR_KEY_REST) ? 1u : 0u,
+                                   e->cfg.out_reverse_endian, s));
+    return PA_OK;
+}
+
+int pa_block_device_split(pa_engine* e, const uint32_t* d_key, const uint32_t* d_seed, uint32_t n, uint32_t m,
+                          uint32_t flags, uint32_t sub_log2, uint8_t* d_out, void* stream, float* d_margin) {
+    if (!e || !d_key || !d_seed || !d_out) return PA_ERR_ARG;
+    std::lock_guard<std::mutex> lk(e->mu);
+    return split_run(e, d_key, d_seed, n, m, flags, sub_log2, reinterpret_cast<uint32_t*>(d_out),
+                     stream_of(e, stream), d_margin, 0, nullptr);
+}
+
+int pa_run_block_split(pa_engine* e, const uint32_t* key, const uint32_t* seed, uint32_t n, uint32_t m,
+                       uint32_t flags, uint32_t sub_log2, uint8_t* out) {
+    if (!e || !key || !seed || !out) return PA_ERR_ARG;
+    std::lock_guard<std::mutex> lk(e->mu);
+    if (n < 64 || (n % 32) || m == 0 || (m % 32) || m >= n || n >= 0x80000000u) return PA_ERR_SIZE;
+    const uint64_t kbytes = align256((uint64_t)(n / 32 + 2) * 4), sbytes = align256((uint64_t)(n / 32) * 4);
+    const uint64_t obytes = align256((uint64_t)m / 8 + 16);
+    hipStream_t s = e->stream;
+    uint8_t* st = nullptr;
+    int rc = split_run(e, nullptr, nullptr, n, m, flags, sub_log2, nullptr, s, nullptr, kbytes + sbytes + obytes, &st);
+    if (rc != PA_OK) return rc;
+    uint32_t* dk = reinterpret_cast<uint32_t*>(st);
+    uint32_t* ds = reinterpret_cast<uint32_t*>(st + kbytes);
+    uint32_t* dout = reinterpret_cast<uint32_t*>(st + kbytes + sbytes);
+    const uint32_t kw = (flags & PA_XOR_KEY_REST) ? n / 32 + 1 : (n - m) / 32 + 1;
+    PA_HIP(hipMemsetAsync(dk, 0, kbytes, s));
+    PA_HIP(hipMemcpyAsync(dk, key, (size_t)kw * 4, hipMemcpyHostToDevice, s));
+    PA_HIP(hipMemcpyAsync(ds, seed, (size_t)(n / 32) * 4, hipMemcpyHostToDevice, s));
+    rc = split_run(e, dk, ds, n, m, flags, sub_log2, dout, s, nullptr, kbytes + sbytes + obytes, nullptr);
+    if (rc != PA_OK) return rc;
+    PA_HIP(hipMemcpyAsync(out, dout, m / 8, hipMemcpyDeviceToHost, s));
     PA_HIP(hipStreamSynchronize(s));
     return PA_OK;
 }

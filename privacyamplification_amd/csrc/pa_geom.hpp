@@ -11,7 +11,8 @@
 //                       16 columns per workgroup, input from the bit-transposed key/seed;
 //   pass B  (rows)      row FFTs of length N2 on rows k1 and N1-k1, real split, DC zeroing, complex
 //                       product, inverse split, inverse row FFTs;
-//   pass A' (inv_cols)  inverse column FFTs of length N1, W columns per workgroup, round-mod-2 epilogue.
+//   pass A' (inv_cols)  inverse column FFTs of length N1, W columns per workgroup, round-mod-2 epilogue
+//                       (N1 >= 2^14: one column per workgroup, 2-bit crumbs instead of nibbles).
 // N <= 8192 (L <= 2^14) runs fully on chip in one workgroup (small path).
 #pragma once
 #include <hip/hip_runtime.h>
@@ -21,7 +22,7 @@ namespace pa {
 
 constexpr int ROW_LOG = 13;            // N2 = 8192 for the four-step path
 constexpr int SMALL_MAX_LOG_N = 13;    // N <= 8192 -> single-workgroup path
-constexpr int MAX_L_LOG = 27;          // largest transform served (L = 2^27)
+constexpr int MAX_L_LOG = 29;          // largest transform served (L = 2^29; the reference stops at 2^27)
 
 struct Geom {
     uint32_t n, m, h, kbits;           // block shape; kbits = h + 1

@@ -124,7 +124,7 @@ __global__ __launch_bounds__(256) void k_transpose_bits(const uint32_t* __restri
 // The class sum over v is four 16-entry table lookups per operand half (bits of tb).
 // Plan 16 x 32 for M = 512: stage 1 computed in registers from the bit masks, stage 2 written
 // straight to Y (thread t: column t mod 16, butterfly t / 16, rows k1' = j + 16 r).
-template <int LGM, int D, bool BATCH>
+template <int LGM, int D, bool BATCH, int CPW = fwd_cols_width(LGM)>
 __global__ __launch_bounds__(FWD_NT) void k_fwd_cols(const uint32_t* __restrict__ tb,
                                                      float2* __restrict__ Y, Geom g,
                                                      const float2* __restrict__ tw14,
@@ -141,13 +141,14 @@ __global__ __launch_bounds__(FWD_NT) void k_fwd_cols(const uint32_t* __restrict_
     constexpr int M = 1 << LGM;
     constexpr int NT = FWD_NT;
     constexpr int SEQS = seq_stride(M);
-    __shared__ float2 buf[16 * SEQS];
+    static_assert(CPW == 16 || LGM != 9, "the fused M = 512 plan needs 16 columns");
+    __shared__ float2 buf[CPW * SEQS];
     __shared__ float2 twl[TWL_SIZE];
     constexpr int NQ = D < 4 ? 1 : D / 4;  // 4-bit tables: v in [4q, 4q+4)
     __shared__ float2 TQ[NQ][16];         // TQ[q][nib] = sum_b bit_b(nib) w_D^{(4q+b) c}
     __shared__ float2 TU[M];              // w_N1^{u c}
     const uint32_t logical = xcd_logical(blockIdx.x, gridDim.x);
-    const uint32_t c = logical % D, grp = logical / D;
+    const uint32_t c = logical % D, grp = logical / D;  // columns [CPW grp, CPW grp + CPW)
     load_twl(twl, tw14, NT);
     for (int e = threadIdx.x; e < NQ * 16; e += NT) {
         const int q = e / 16, nib = e % 16;
@@ -158,8 +159,14 @@ __global__ __launch_bounds__(FWD_NT) void k_fwd_cols(const uint32_t* __restrict_
         }
         TQ[q][nib] = acc;
     }
-    const uint32_t step_n1 = TW14 / g.N1;
-    for (int u = threadIdx.x; u < M; u += NT) TU[u] = tw14[((uint32_t)u * c) * step_n1];
+    // w_N1^{u c}: from the 2^14 table while N1 <= 2^14, else from the length-L two-level table
+    if (g.N1 <= (uint32_t)TW14) {
+        const uint32_t step_n1 = TW14 / g.N1;
+        for (int u = threadIdx.x; u < M; u += NT) TU[u] = tw14[((uint32_t)u * c) * step_n1];
+    } else {
+        const uint32_t step_l = g.L / g.N1;
+        for (int u = threadIdx.x; u < M; u += NT) TU[u] = tw_two_level(twlo, twhi, ((uint32_t)u * c) * step_l);
+    }
     // the element z(col, u) of this class (both halves of tb: even and odd bits)
     auto zval = [&](uint32_t t, int u) {
         float2 sE = TQ[0][t & 15u], sO = TQ[0][(t >> 16) & 15u];
@@ -216,24 +223,24 @@ __global__ __launch_bounds__(FWD_NT) void k_fwd_cols(const uint32_t* __restrict_
             tw = cmul(tw, step);
         }
     } else {
-        constexpr int IPT = (16 * M + NT - 1) / NT;
+        constexpr int IPT = (CPW * M + NT - 1) / NT;
         uint32_t tbv[IPT];
 #pragma unroll
         for (int i = 0; i < IPT; ++i) {
             const int e = threadIdx.x + i * NT;
-            tbv[i] = (e < 16 * M) ? tb[(uint32_t)(e >> 4) * g.N2 + grp * 16u + (uint32_t)(e & 15)] : 0u;
+            tbv[i] = (e < CPW * M) ? tb[(uint32_t)(e / CPW) * g.N2 + grp * CPW + (uint32_t)(e % CPW)] : 0u;
         }
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < IPT; ++i) {
             const int e = threadIdx.x + i * NT;
-            if (e < 16 * M) buf[(e & 15) * SEQS + padi(e >> 4)] = zval(tbv[i], e >> 4);
+            if (e < CPW * M) buf[(e % CPW) * SEQS + padi(e / CPW)] = zval(tbv[i], e / CPW);
         }
         __syncthreads();
-        lds_fft<LGM, 16, SEQS, NT>(buf, twl);
-        for (int e = threadIdx.x; e < 16 * M; e += NT) {
-            const int col = e & 15, k1p = e >> 4;
-            const uint32_t k1 = c + (uint32_t)D * (uint32_t)k1p, t2 = grp * 16u + (uint32_t)col;
+        lds_fft<LGM, CPW, SEQS, NT>(buf, twl);
+        for (int e = threadIdx.x; e < CPW * M; e += NT) {
+            const int col = e % CPW, k1p = e / CPW;
+            const uint32_t k1 = c + (uint32_t)D * (uint32_t)k1p, t2 = grp * CPW + (uint32_t)col;
             const float2 tw = tw_two_level(twlo, twhi, (uint32_t)((2ull * t2 * k1) & (g.L - 1u)));
             Y[(size_t)k1 * g.N2 + t2] = cmul(buf[col * SEQS + padi(k1p)], tw);
         }
@@ -484,7 +491,8 @@ template <int LG1, bool BATCH>
 __global__ __launch_bounds__(inv_cols_threads(LG1)) void k_inv_cols(
     const float2* __restrict__ Rp, uint8_t* __restrict__ nibT, Geom g,
     const uint32_t* __restrict__ wk_cnt, const uint32_t* __restrict__ ws_cnt,
-    const float2* __restrict__ tw14, float* __restrict__ margin, const BlockArgs* __restrict__ bt) {
+    const float2* __restrict__ tw14, const float2* __restrict__ twlo, const float2* __restrict__ twhi,
+    float* __restrict__ margin, const BlockArgs* __restrict__ bt) {
     if constexpr (BATCH) {
         const BlockArgs& a = bt[blockIdx.z];
         g = a.g;
@@ -497,11 +505,58 @@ __global__ __launch_bounds__(inv_cols_threads(LG1)) void k_inv_cols(
     constexpr int N1 = 1 << LG1;
     constexpr int W = inv_cols_width(LG1);
     constexpr int NT = inv_cols_threads(LG1);
-    constexpr int SEQS = seq_stride(N1);
+    constexpr int LGL = LG1 < 14 ? LG1 : 14;  // LDS transform length (N1 = 2^15: two halves)
+    constexpr int SEQS = seq_stride(1 << LGL);
     __shared__ float2 buf[W * SEQS];
     __shared__ float2 twl[TWL_SIZE];
     load_twl(twl, tw14, NT);
     const uint32_t grp = xcd_logical(blockIdx.x, gridDim.x);
+    if constexpr (LG1 >= 14) {
+        // One column t2 = grp per workgroup; output one 2-bit crumb per t1 at nibT[t2 * N1 + t1]
+        // (bits c[2t], c[2t+1] of t = t2 + N2 t1).  N1 = 2^15 splits the column first by one
+        // radix-2 step in registers (X[2k] = FFT(a)[k], X[2k+1] = FFT(b)[k] with a = x[j] + x[j+N1/2],
+        // b = (x[j] - x[j+N1/2]) w_N1^j), then transforms the two 2^14-point halves in turn.
+        static_assert(W == 1 && NT == 1024, "single-column pass expects 1024 threads");
+        constexpr int H = 1 << LGL;       // points per LDS transform
+        constexpr int PT = N1 / NT;       // values per thread (16 or 32)
+        const uint32_t t2 = grp;
+        const float corr = correction(pop_total(wk_cnt), pop_total(ws_cnt), g.L);
+        const float invN = g.inv_N;
+        float dist = 0.f;
+        float2 v[PT];
+#pragma unroll
+        for (int r = 0; r < PT; ++r) v[r] = Rp[(size_t)(threadIdx.x + NT * r) * g.N2 + t2];
+        __syncthreads();  // twl
+        constexpr int HALVES = N1 / H;
+        if constexpr (HALVES == 2) {
+            const uint32_t step = g.L / (uint32_t)N1;
+#pragma unroll
+            for (int r = 0; r < PT / 2; ++r) {
+                const float2 x0 = v[r], x1 = v[r + PT / 2];
+                v[r] = cadd(x0, x1);
+                v[r + PT / 2] = cmul(csub(x0, x1), tw_two_level(twlo, twhi, (threadIdx.x + NT * r) * step));
+            }
+        }
+#pragma unroll
+        for (int hv = 0; hv < HALVES; ++hv) {
+#pragma unroll
+            for (int r = 0; r < H / NT; ++r) buf[padi(threadIdx.x + NT * r)] = v[hv * (H / NT) + r];
+            __syncthreads();
+            lds_fft<LGL, 1, SEQS, NT>(buf, twl);
+            for (int k = threadIdx.x; k < H; k += NT) {
+                const uint32_t t1 = (uint32_t)(k * HALVES + hv);
+                const uint32_t i0 = 2u * (t2 + g.N2 * t1);
+                if (i0 + 2u <= g.out_off || i0 >= g.out_off + g.out_bits) continue;
+                const float2 G = buf[padi(k)];
+                const uint32_t b0 = round_bit(G.x * invN + corr, dist);
+                const uint32_t b1 = round_bit(-G.y * invN + corr, dist);
+                nibT[(size_t)t2 * N1 + t1] = (uint8_t)((b0 << 1) | b1);
+            }
+            __syncthreads();  // reads done before the next half is written
+        }
+        if (margin) margin_publish(dist, margin);
+        return;
+    } else {
     if constexpr (LG1 == 13) {
         // plan 32 x 16 x 16 over the two columns: stage 1 straight from global memory (thread t:
         // column t & 1, butterfly t >> 1, rows j + 256 r), stage 3 into registers (thread t:
@@ -572,6 +627,7 @@ __global__ __launch_bounds__(inv_cols_threads(LG1)) void k_inv_cols(
         nibT[(size_t)(grp * P + (uint32_t)p) * N1 + (uint32_t)t1] = (uint8_t)((b0 << 3) | (b1 << 2) | (b2 << 1) | b3);
     }
     if (margin) margin_publish(dist, margin);
+    }
 }
 
 // ------------------------------------------------------------------ output ----
@@ -581,7 +637,9 @@ __global__ __launch_bounds__(inv_cols_threads(LG1)) void k_inv_cols(
 // [out_off + 32w, +32), XOR key_rest word w ((key[h/32+w] << 1) | (key[h/32+w+1] >> 31),
 // key2StartRest PrivacyAmplification.cu:873-878), byte-swapped for the OutBytes layout
 // (ToBinaryArray, PrivacyAmplification.cu:757-762).
-template <bool BATCH>
+// CRUMB: pass A' wrote one 2-bit crumb per column, crumb[t2 * N1 + t1]; the nibble of pair p is
+// crumb[2p] << 2 | crumb[2p + 1].
+template <bool BATCH, bool CRUMB>
 __global__ __launch_bounds__(256) void k_pack(const uint8_t* __restrict__ nibT,
                                               const uint32_t* __restrict__ key,
                                               uint32_t* __restrict__ out, Geom g,
@@ -603,7 +661,15 @@ __global__ __launch_bounds__(256) void k_pack(const uint8_t* __restrict__ nibT,
         const uint32_t gp = e / (PACK_T1 / 4), r4 = (e % (PACK_T1 / 4)) * 4;
         const uint32_t row = row0 + r4;
         uint32_t v = 0;
-        if (row < g.N1) v = *reinterpret_cast<const uint32_t*>(nibT + (size_t)(g0 + gp) * g.N1 + row);
+        if constexpr (CRUMB) {
+            if (row < g.N1) {
+                const uint32_t ve = *reinterpret_cast<const uint32_t*>(nibT + (size_t)(2u * (g0 + gp)) * g.N1 + row);
+                const uint32_t vo = *reinterpret_cast<const uint32_t*>(nibT + (size_t)(2u * (g0 + gp) + 1u) * g.N1 + row);
+                v = ((ve & 0x03030303u) << 2) | (vo & 0x03030303u);
+            }
+        } else {
+            if (row < g.N1) v = *reinterpret_cast<const uint32_t*>(nibT + (size_t)(g0 + gp) * g.N1 + row);
+        }
 #pragma unroll
         for (int k = 0; k < 4; ++k) tile[r4 + k][gp] = (uint8_t)(v >> (8 * k));
     }
@@ -763,11 +829,83 @@ __global__ __launch_bounds__(small_threads(LGN)) void k_small(
     if (margin) margin_publish(dist, margin);
 }
 
+// ------------------------------------------------------------------ sub-block split ----
+// The m x (h+1) Toeplitz product cut into b x b tiles (the tiling of LargeBlocksizeExample,
+// LBS/main.cpp:104-124, 313-523): tile (band i, key column j) is the length-2b circular product of
+// key bits [jb, jb+b) (bit b zero) with the seed segment S[R, R+b) || S[R-b, R) at R = (i-j)b mod n,
+// read at [0, b): z[v] = XOR_u x[jb+u] s[((i-j)b + v - u) mod n].  Tiles of a chunk are consecutive
+// key columns j0.., all bands each: tile t = (j - j0) * nbands + i.
+__global__ __launch_bounds__(256) void k_tile_inputs(const uint32_t* __restrict__ key,
+                                                     const uint32_t* __restrict__ seed, uint32_t n,
+                                                     uint32_t h, uint32_t b, uint32_t nbands, uint32_t j0,
+                                                     uint32_t* __restrict__ tkeys, uint32_t* __restrict__ tseeds) {
+    const uint32_t t = blockIdx.y, i = t % nbands, j = j0 + t / nbands;
+    const uint32_t wb = b >> 5, nw = n >> 5, hw = h >> 5;
+    const uint32_t R = (uint32_t)(((int64_t)i - (int64_t)j) * (int64_t)b % (int64_t)n + (int64_t)n) % n;
+    const uint32_t R2 = (R + n - b % n) % n;
+    uint32_t* tk = tkeys + (size_t)t * (wb + 1);
+    uint32_t* ts = tseeds + (size_t)t * (2 * wb);
+    for (uint32_t w = blockIdx.x * 256u + threadIdx.x; w < 3u * wb + 1u; w += gridDim.x * 256u) {
+        if (w <= wb) {  // key words; word wb (bit b) is zero
+            const uint32_t kw = j * wb + w;
+            uint32_t v = 0;
+            if (w < wb) v = kw < hw ? key[kw] : (kw == hw ? key[kw] & 0x80000000u : 0u);
+            tk[w] = v;
+        } else {
+            const uint32_t sw = w - wb - 1u;  // seed word of the tile, < 2 wb
+            const uint32_t src = sw < wb ? (R >> 5) + sw : (R2 >> 5) + (sw - wb);
+            ts[sw] = seed[src % nw];
+        }
+    }
+}
+
+// acc[w] (=|^=) XOR over the chunk's columns of the tile outputs of band w / wb (32-bit words of
+// the engine's output layout, which XOR word for word).
+__global__ __launch_bounds__(256) void k_tile_xor(uint32_t* __restrict__ acc, const uint32_t* __restrict__ tout,
+                                                  uint32_t nbands, uint32_t wb, uint32_t cols, int first) {
+    const uint32_t w = blockIdx.x * 256u + threadIdx.x;
+    if (w >= nbands * wb) return;
+    const uint32_t i = w / wb, ww = w % wb;
+    uint32_t x = first ? 0u : acc[w];
+    for (uint32_t c = 0; c < cols; ++c) x ^= tout[(size_t)(c * nbands + i) * wb + ww];
+    acc[w] = x;
+}
+
+// out word w = acc word w XOR the key rest word ((key[h/32+w] << 1) | (key[h/32+w+1] >> 31),
+// key2StartRest PrivacyAmplification.cu:873-878) in the output byte order.
+__global__ __launch_bounds__(256) void k_split_finish(const uint32_t* __restrict__ acc,
+                                                      const uint32_t* __restrict__ key, uint32_t* __restrict__ out,
+                                                      uint32_t m, uint32_t h, uint32_t do_xor, uint32_t reverse_endian) {
+    const uint32_t w = blockIdx.x * 256u + threadIdx.x;
+    if (w >= (m >> 5)) return;
+    uint32_t kr = 0;
+    if (do_xor) kr = (key[(h >> 5) + w] << 1) | (key[(h >> 5) + w + 1] >> 31);
+    out[w] = acc[w] ^ (reverse_endian ? __builtin_bswap32(kr) : kr);
+}
+
+hipError_t launch_tile_inputs(const uint32_t* key, const uint32_t* seed, uint32_t n, uint32_t h, uint32_t b,
+                              uint32_t nbands, uint32_t j0, uint32_t tiles, uint32_t* tkeys, uint32_t* tseeds,
+                              hipStream_t s) {
+    const uint32_t words = 3u * (b >> 5) + 1u;
+    k_tile_inputs<<<dim3((words + 255u) / 256u, tiles), 256, 0, s>>>(key, seed, n, h, b, nbands, j0, tkeys, tseeds);
+    return hipGetLastError();
+}
+hipError_t launch_tile_xor(uint32_t* acc, const uint32_t* tout, uint32_t nbands, uint32_t wb, uint32_t cols,
+                           int first, hipStream_t s) {
+    k_tile_xor<<<(nbands * wb + 255u) / 256u, 256, 0, s>>>(acc, tout, nbands, wb, cols, first);
+    return hipGetLastError();
+}
+hipError_t launch_split_finish(const uint32_t* acc, const uint32_t* key, uint32_t* out, uint32_t m, uint32_t h,
+                               uint32_t do_xor, uint32_t reverse_endian, hipStream_t s) {
+    k_split_finish<<<((m >> 5) + 255u) / 256u, 256, 0, s>>>(acc, key, out, m, h, do_xor, reverse_endian);
+    return hipGetLastError();
+}
+
 // ================================================================== launchers ====
 namespace {
 template <int LGM, int D>
 hipError_t launch_fwd_cols_t(const uint32_t* tb, float2* Y, const Geom& g, const Tables& t, hipStream_t s) {
-    const uint32_t grid = (g.N2 / 16u) * (uint32_t)D;
+    const uint32_t grid = (g.N2 / (uint32_t)fwd_cols_width(LGM)) * (uint32_t)D;
     k_fwd_cols<LGM, D, false><<<grid, FWD_NT, 0, s>>>(tb, Y, g, t.tw14, t.lo, t.hi, nullptr, 0);
     return hipGetLastError();
 }
@@ -775,7 +913,7 @@ template <int LG1>
 hipError_t launch_inv_cols_t(const float2* R, uint8_t* nib, const Geom& g, const uint32_t* wk,
                              const uint32_t* ws, const Tables& t, float* margin, hipStream_t s) {
     const uint32_t grid = g.N2 / (uint32_t)inv_cols_width(LG1);
-    k_inv_cols<LG1, false><<<grid, inv_cols_threads(LG1), 0, s>>>(R, nib, g, wk, ws, t.tw14, margin, nullptr);
+    k_inv_cols<LG1, false><<<grid, inv_cols_threads(LG1), 0, s>>>(R, nib, g, wk, ws, t.tw14, t.lo, t.hi, margin, nullptr);
     return hipGetLastError();
 }
 template <int D>
@@ -832,6 +970,8 @@ hipError_t launch_fwd_cols(const uint32_t* tb, float2* Y, const Geom& g, const T
     case 11: return launch_fwd_cols_t<9, 4>(tb, Y, g, t, s);
     case 12: return launch_fwd_cols_t<9, 8>(tb, Y, g, t, s);
     case 13: return launch_fwd_cols_t<9, 16>(tb, Y, g, t, s);
+    case 14: return launch_fwd_cols_t<10, 16>(tb, Y, g, t, s);
+    case 15: return launch_fwd_cols_t<11, 16>(tb, Y, g, t, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -864,6 +1004,8 @@ hipError_t launch_inv_cols(const float2* R, uint8_t* nib, const Geom& g, const u
     case 11: return launch_inv_cols_t<11>(R, nib, g, wk, ws, t, margin, s);
     case 12: return launch_inv_cols_t<12>(R, nib, g, wk, ws, t, margin, s);
     case 13: return launch_inv_cols_t<13>(R, nib, g, wk, ws, t, margin, s);
+    case 14: return launch_inv_cols_t<14>(R, nib, g, wk, ws, t, margin, s);
+    case 15: return launch_inv_cols_t<15>(R, nib, g, wk, ws, t, margin, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -873,7 +1015,8 @@ hipError_t launch_pack(const uint8_t* nibT, const uint32_t* key, uint32_t* out, 
     const uint32_t q_lo = g.out_off >> 2, q_hi = (g.out_off + g.out_bits) >> 2;
     const uint32_t rows = (q_hi - 1u) / P2 - ((q_lo / P2) & ~3u) + 1u;
     dim3 grid(P2 / PACK_G, (rows + PACK_T1 - 1u) / PACK_T1);
-    k_pack<false><<<grid, 256, 0, s>>>(nibT, key, out, g, nullptr);
+    if (g.W == 1) k_pack<false, true><<<grid, 256, 0, s>>>(nibT, key, out, g, nullptr);
+    else k_pack<false, false><<<grid, 256, 0, s>>>(nibT, key, out, g, nullptr);
     return hipGetLastError();
 }
 
@@ -920,14 +1063,14 @@ hipError_t batch_transpose_t(const BlockArgs* bt, uint32_t count, const Geom& g0
 template <int LGM, int D>
 hipError_t batch_fwd_cols_t(const BlockArgs* bt, uint32_t count, const Geom& g0, const Tables& t, int is_seed,
                             hipStream_t s) {
-    k_fwd_cols<LGM, D, true><<<dim3((g0.N2 / 16u) * (uint32_t)D, 1, count), FWD_NT, 0, s>>>(
+    k_fwd_cols<LGM, D, true><<<dim3((g0.N2 / (uint32_t)fwd_cols_width(LGM)) * (uint32_t)D, 1, count), FWD_NT, 0, s>>>(
         nullptr, nullptr, g0, t.tw14, t.lo, t.hi, bt, is_seed);
     return hipGetLastError();
 }
 template <int LG1>
 hipError_t batch_inv_cols_t(const BlockArgs* bt, uint32_t count, const Geom& g0, const Tables& t, hipStream_t s) {
     k_inv_cols<LG1, true><<<dim3(g0.N2 / (uint32_t)inv_cols_width(LG1), 1, count), inv_cols_threads(LG1), 0, s>>>(
-        nullptr, nullptr, g0, nullptr, nullptr, t.tw14, nullptr, bt);
+        nullptr, nullptr, g0, nullptr, nullptr, t.tw14, t.lo, t.hi, nullptr, bt);
     return hipGetLastError();
 }
 template <int LGN>
@@ -965,6 +1108,8 @@ hipError_t launch_batch_fwd_cols(const BlockArgs* bt, uint32_t count, const Geom
     case 11: return batch_fwd_cols_t<9, 4>(bt, count, g0, t, is_seed, s);
     case 12: return batch_fwd_cols_t<9, 8>(bt, count, g0, t, is_seed, s);
     case 13: return batch_fwd_cols_t<9, 16>(bt, count, g0, t, is_seed, s);
+    case 14: return batch_fwd_cols_t<10, 16>(bt, count, g0, t, is_seed, s);
+    case 15: return batch_fwd_cols_t<11, 16>(bt, count, g0, t, is_seed, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -997,12 +1142,16 @@ hipError_t launch_batch_inv_cols(const BlockArgs* bt, uint32_t count, const Geom
     case 11: return batch_inv_cols_t<11>(bt, count, g0, t, s);
     case 12: return batch_inv_cols_t<12>(bt, count, g0, t, s);
     case 13: return batch_inv_cols_t<13>(bt, count, g0, t, s);
+    case 14: return batch_inv_cols_t<14>(bt, count, g0, t, s);
+    case 15: return batch_inv_cols_t<15>(bt, count, g0, t, s);
     default: return hipErrorInvalidValue;
     }
 }
 
 hipError_t launch_batch_pack(const BlockArgs* bt, uint32_t count, const Geom& g0, uint32_t tiles_y, hipStream_t s) {
-    k_pack<true><<<dim3((g0.N2 >> 1) / PACK_G, tiles_y, count), 256, 0, s>>>(nullptr, nullptr, nullptr, g0, bt);
+    const dim3 grid((g0.N2 >> 1) / PACK_G, tiles_y, count);
+    if (g0.W == 1) k_pack<true, true><<<grid, 256, 0, s>>>(nullptr, nullptr, nullptr, g0, bt);
+    else k_pack<true, false><<<grid, 256, 0, s>>>(nullptr, nullptr, nullptr, g0, bt);
     return hipGetLastError();
 }
 

@@ -25,12 +25,17 @@ struct Tables {
 
 // Columns per workgroup in pass A' (tile of W x N1 complex <= 16384 values) and its threads.
 __host__ __device__ constexpr int inv_cols_width(int lg1) {
-    return (14 - lg1) >= 4 ? 16 : (1 << (14 - lg1));
+    return lg1 >= 14 ? 1 : ((14 - lg1) >= 4 ? 16 : (1 << (14 - lg1)));
 }
 __host__ __device__ constexpr int clamp_threads(int t) { return t < 64 ? 64 : (t > 1024 ? 1024 : t); }
 __host__ __device__ constexpr int inv_cols_threads(int lg1) {
-    return lg1 == 13 ? 512 : clamp_threads(inv_cols_width(lg1) * (1 << lg1) / 16);
+    return lg1 == 13 ? 512 : (lg1 >= 14 ? 1024 : clamp_threads(inv_cols_width(lg1) * (1 << lg1) / 16));
 }
+// Columns per workgroup in pass A (16; 8 when the class length M = 2048 would overflow LDS).
+__host__ __device__ constexpr int fwd_cols_width(int lgm) { return lgm >= 11 ? 8 : 16; }
+// Bytes of the pass-A' output: nibbles per column pair (N/2), or 2-bit crumbs per column (N) when
+// a workgroup holds a single column.
+__host__ __device__ constexpr uint64_t nib_bytes(uint32_t N, uint32_t W) { return (W == 1 ? (uint64_t)N : N / 2) + 64; }
 __host__ __device__ constexpr int small_threads(int lgn) { return clamp_threads(2 * (1 << lgn) / 16); }
 
 hipError_t launch_zero_u32(uint32_t* p, uint32_t count, hipStream_t s);
@@ -62,5 +67,14 @@ hipError_t launch_batch_rows(int mode, const BlockArgs* bt, uint32_t count, cons
 hipError_t launch_batch_inv_cols(const BlockArgs* bt, uint32_t count, const Geom& g0, const Tables& t, hipStream_t s);
 hipError_t launch_batch_pack(const BlockArgs* bt, uint32_t count, const Geom& g0, uint32_t tiles_y, hipStream_t s);
 hipError_t launch_batch_small(const BlockArgs* bt, uint32_t count, const Geom& g0, const Tables& t, hipStream_t s);
+
+// Sub-block split (tiles of b x b through the batched engine, XOR-accumulated per band).
+hipError_t launch_tile_inputs(const uint32_t* key, const uint32_t* seed, uint32_t n, uint32_t h, uint32_t b,
+                              uint32_t nbands, uint32_t j0, uint32_t tiles, uint32_t* tkeys, uint32_t* tseeds,
+                              hipStream_t s);
+hipError_t launch_tile_xor(uint32_t* acc, const uint32_t* tout, uint32_t nbands, uint32_t wb, uint32_t cols,
+                           int first, hipStream_t s);
+hipError_t launch_split_finish(const uint32_t* acc, const uint32_t* key, uint32_t* out, uint32_t m, uint32_t h,
+                               uint32_t do_xor, uint32_t reverse_endian, hipStream_t s);
 
 }  // namespace pa

@@ -128,6 +128,10 @@ def load_library():
                                          ctypes.c_uint32, vp, vp, vp]
     L.pa_batch_device.argtypes = [eng, ctypes.POINTER(PaBlock), ctypes.c_uint32, ctypes.c_uint32, vp]
     L.pa_run_batch.argtypes = [eng, ctypes.POINTER(PaBlock), ctypes.c_uint32, ctypes.c_uint32]
+    L.pa_block_device_split.argtypes = [eng, vp, vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                        ctypes.c_uint32, vp, vp, vp]
+    L.pa_run_block_split.argtypes = [eng, u32p, u32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                     ctypes.c_uint32, u8p]
     L.pa_host_alloc.argtypes = [eng, ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]
     L.pa_host_free.argtypes = [eng, ctypes.c_void_p]
     L.pa_serve_start.argtypes = [eng, ctypes.POINTER(PaServeConfig)]
@@ -147,7 +151,8 @@ def load_library():
                  "pa_run_block", "pa_run_block_seeded", "pa_submit", "pa_wait", "pa_block_device",
                  "pa_block_device_seeded", "pa_synth_bits", "pa_stage_timing", "pa_stage_times",
                  "pa_batch_device", "pa_run_batch", "pa_serve_start", "pa_serve_submit",
-                 "pa_serve_wait", "pa_serve_stop", "pa_host_alloc", "pa_host_free"):
+                 "pa_serve_wait", "pa_serve_stop", "pa_host_alloc", "pa_host_free",
+                 "pa_block_device_split", "pa_run_block_split"):
         getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L
@@ -303,6 +308,19 @@ class Engine:
         else:
             arr = blocks
         _check(self._L.pa_batch_device(self._h, arr, len(arr), batch_flags, stream), "pa_batch_device")
+
+    def amplify_split(self, key, seed, n, m, sub_log2, do_xor_key_rest=True):
+        """Sub-block (tiled, b = 2^sub_log2) evaluation of one block (pa_run_block_split)."""
+        out = np.zeros(m // 8 + 8, dtype=np.uint8)
+        k, kp = _u32(key)
+        s, sp = _u32(seed)
+        _check(self._L.pa_run_block_split(self._h, kp, sp, n, m, self.flags(do_xor_key_rest, True), sub_log2,
+                                          out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))), "pa_run_block_split")
+        return out[: m // 8]
+
+    def block_device_split(self, d_key, d_seed, n, m, flags, sub_log2, d_out, stream=None, d_margin=None):
+        _check(self._L.pa_block_device_split(self._h, d_key, d_seed, n, m, flags, sub_log2, d_out, stream, d_margin),
+               "pa_block_device_split")
 
     def host_empty(self, count, dtype=np.uint8):
         """A numpy array in pinned host memory (pa_host_alloc); freed when the array is collected.

@@ -4,7 +4,8 @@ tests and the bench use at sizes too large to recompute inside a test run.
 Inputs follow the measurement plan: xoshiro256** seeded per block, Bernoulli(0.5) bits, key of
 n+1 bits, seed of n bits, m = 3n/8, XOR and compress on.  Output digests come from the exact NTT
 oracle (oracle/pa_oracle.c, cross-checked against the direct GF(2) product at smaller n).
-Run: python tests/golden/make_manifest.py   (about 2 minutes on 8 cores)
+Run: python tests/golden/make_manifest.py   (missing entries only; the 2^28 and 2^29 blocks take
+several minutes each on 8 cores)
 """
 import hashlib
 import json
@@ -28,20 +29,33 @@ def entry(name, n, m, key_seed, toeplitz_seed, p_key=0.5):
             "sha3": hashlib.sha3_256(out.tobytes()).hexdigest(), "out_bytes": int(out.size)}
 
 
-def main():
-    O.build()
-    items = [entry("pr1_n20", 1 << 20, 3 << 17, 0x11, 0x22)]
+def plan():
+    """(name, n, m, key_seed, toeplitz_seed, p_key) of every manifest block."""
+    items = [("pr1_n20", 1 << 20, 3 << 17, 0x11, 0x22, 0.5)]
     n = 1 << 27
     m = 3 * n // 8
     for i in range(2):
-        items.append(entry(f"c2_dynamic_{i}", n, m, 0x1000 + i, 0x2000 + i))
-        items.append(entry(f"c2_static_{i}", n, m, 0x1000 + i, 0x5EED))
-    for e in (24, 26):
+        items.append((f"c2_dynamic_{i}", n, m, 0x1000 + i, 0x2000 + i, 0.5))
+        items.append((f"c2_static_{i}", n, m, 0x1000 + i, 0x5EED, 0.5))
+    # precision sweep sizes; 2^28 and 2^29 exceed the reference's largest block (oracle-only parity)
+    for e in (24, 26, 28, 29):
         nn = 1 << e
         for pk in (0.5, 0.9):
-            items.append(entry(f"c3_n{e}_p{pk}", nn, 3 * nn // 8, 0x3000 + e, 0x4000 + e, pk))
+            items.append((f"c3_n{e}_p{pk}", nn, 3 * nn // 8, 0x3000 + e, 0x4000 + e, pk))
+    return items
+
+
+def main():
+    """Computes the entries missing from manifest.json (all of them with --all)."""
+    O.build()
     out = pathlib.Path(__file__).with_name("manifest.json")
-    out.write_text(json.dumps({"generator": "oracle NTT (exact)", "blocks": items}, indent=1) + "\n")
+    have = {}
+    if out.exists() and "--all" not in sys.argv:
+        have = {b["name"]: b for b in json.loads(out.read_text())["blocks"]}
+    items = []
+    for name, n, m, ks, ts, pk in plan():
+        items.append(have[name] if name in have else entry(name, n, m, ks, ts, pk))
+        out.write_text(json.dumps({"generator": "oracle NTT (exact)", "blocks": items}, indent=1) + "\n")
     print("wrote", out)
 
 

@@ -55,7 +55,8 @@ def test_workspace_sizes_and_rejections():
     assert P.workspace_bytes(1000, 96) == 0          # not a multiple of 32
     assert P.workspace_bytes(1024, 1024) == 0        # m >= n
     assert P.workspace_bytes(3 << 20, 3 << 18, P.XOR_KEY_REST) == 0  # non-2^k without compression
-    assert P.workspace_bytes(1 << 28, 3 << 25) == 0  # beyond the largest served transform
+    assert P.workspace_bytes(1 << 30, 3 << 27) == 0  # beyond the largest served transform (2^29)
+    assert 4 << 30 < P.workspace_bytes(1 << 29, 3 << 26) < 6 << 30
 
 
 def test_no_cpu_fallback_when_library_missing(tmp_path, monkeypatch):

@@ -80,6 +80,35 @@ def test_ntt_any_length_matches_direct(oracle, n, m):
                               oracle.amp_out(k, seed, n, m, do_xor, True, method=oracle.NTT))
 
 
+@pytest.mark.parametrize("n,kbits,nout", [(256, 161, 96), (480, 200, 280), (1024, 700, 512)])
+def test_long_length_ntt_against_brute_force(oracle, n, kbits, nout):
+    """The NTT over 3*2^30+1 (used for transforms longer than 2^27) forced on small cases."""
+    rng = np.random.default_rng(n * 11 + kbits)
+    ks = rng.integers(0, 2**32, n // 32, dtype=np.uint64).astype(np.uint32)
+    seed = rng.integers(0, 2**32, n // 32, dtype=np.uint64).astype(np.uint32)
+    oracle.set_ntt_big(True)
+    try:
+        got = oracle.conv_bits(ks, kbits, seed, n, nout, oracle.NTT)
+        counts = oracle.conv_counts(ks, kbits, seed, n, nout) if n & (n - 1) == 0 else None
+    finally:
+        oracle.set_ntt_big(False)
+    assert _bits_msb(got, nout) == _brute(ks, kbits, seed, n, nout)
+    if counts is not None:
+        assert np.array_equal(counts, oracle.conv_counts(ks, kbits, seed, n, nout))
+
+
+def test_long_length_ntt_matches_direct(oracle):
+    n, m = 1 << 16, 3 << 13
+    key = oracle.synth_bits(0x31, n + 1)
+    seed = oracle.synth_bits(0x32, n)
+    oracle.set_ntt_big(True)
+    try:
+        got = oracle.amp_out(key, seed, n, m, method=oracle.NTT)
+    finally:
+        oracle.set_ntt_big(False)
+    assert np.array_equal(got, oracle.amp_out(key, seed, n, m, method=oracle.DIRECT))
+
+
 @pytest.mark.parametrize("e", [10, 12, 16, 20])
 def test_methods_agree(oracle, e):
     n = 1 << e
