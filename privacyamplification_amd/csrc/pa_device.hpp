@@ -46,6 +46,13 @@ __device__ __forceinline__ float2 twc(float2 v, int k) {
     return cmul(v, make_float2(w32_re(k32), w32_im(k32)));
 }
 
+// v * exp(-2 pi i r / 32) for a compile-time-foldable r in [0, 32).
+__device__ __forceinline__ float2 w32pow(float2 v, int r) {
+    if (r < 16) return twc<32>(v, r);
+    const float2 t = twc<32>(v, r - 16);
+    return make_float2(-t.x, -t.y);
+}
+
 // In-register forward DFT, natural-order input and output (radix-2 decimation in time).
 template <int R>
 __device__ __forceinline__ void dft(float2 (&x)[R]) {
@@ -138,9 +145,27 @@ __device__ __forceinline__ int opaque_int(int x) {
     return x;
 }
 
+// Twiddle table of one Stockham stage in LDS: row q < NS holds w^(q r / (NS R)) for r < R, rows
+// padded to R + 1 entries (bank spread).  Filled by all threads; the caller synchronises.
+template <int R, int NS>
+__device__ __forceinline__ void fill_stage_table(float2* tab, const float2* __restrict__ tw14, int nthreads) {
+    for (int e = threadIdx.x; e < NS * (R + 1); e += nthreads) {
+        const int q = e / (R + 1), r = e % (R + 1);
+        tab[e] = r < R ? tw14[(q * (TW14 / (NS * R)) * r) & (TW14 - 1)] : make_float2(0.f, 0.f);
+    }
+}
+
+// Hides a complex value from the optimiser so that quantities derived from it are computed where
+// they are used instead of being hoisted and kept live.
+__device__ __forceinline__ float2 opaque_f2(float2 v) {
+    asm volatile("" : "+v"(v.x), "+v"(v.y));
+    return v;
+}
+
 // One in-place Stockham radix-R stage over NSEQ sequences of length 2^LG in LDS.
-// Reads all butterflies' inputs, barrier, writes, barrier.
-template <int R, int NS, int LG, int NSEQ, int SEQS, int NT>
+// Reads all butterflies' inputs, barrier, writes, barrier.  TAB: twl is this stage's table
+// (fill_stage_table) instead of the generic two-level table.
+template <int R, int NS, int LG, int NSEQ, int SEQS, int NT, bool TAB = false>
 __device__ __forceinline__ void stockham_stage(float2* buf, const float2* twl) {
     constexpr int L = 1 << LG;
     constexpr int BPS = L / R;
@@ -163,7 +188,11 @@ __device__ __forceinline__ void stockham_stage(float2* buf, const float2* twl) {
 #pragma unroll
                 for (int r = 0; r < R; ++r) v[i][r] = base[padi(j + r * BPS)];
             }
-            if constexpr (NS > 1) {
+            if constexpr (NS > 1 && TAB) {
+                const float2* trow = twl + (j % NS) * (R + 1);
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[i][r] = cmul(v[i][r], trow[r]);
+            } else if constexpr (NS > 1) {
                 float2 tw[R];
                 twiddle_powers<R>(tw, twl, (j % NS) * (TW14 / (NS * R)));
 #pragma unroll

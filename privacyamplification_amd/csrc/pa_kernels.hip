@@ -304,7 +304,11 @@ __global__ __launch_bounds__(ROWS_NT) void k_rows(float2* __restrict__ Yk, float
     static_assert(2 * N2 == TW14, "row split twiddle assumes N2 = 2^13");
     __shared__ float2 buf[2 * SEQS];
     __shared__ float2 twl[TWL_SIZE];
+    __shared__ float2 ttf[32 * 17];  // forward middle stage (radix 16, span 32)
+    __shared__ float2 tti[16 * 17];  // inverse middle stage (radix 16, span 16)
     load_twl(twl, tw14, NT);
+    fill_stage_table<16, 32>(ttf, tw14, NT);
+    fill_stage_table<16, 16>(tti, tw14, NT);
     const uint32_t N1 = g.N1, items = N1 >> 1;
     const int tid = threadIdx.x;
     // stage-1 butterfly of this thread: row (t >> 8), j = t & 255, inputs j + 256 r
@@ -323,7 +327,7 @@ __global__ __launch_bounds__(ROWS_NT) void k_rows(float2* __restrict__ Yk, float
         stage_first_to_lds<32>(v, buf + s1seq * SEQS, s1j);
         refill(v);
         __syncthreads();
-        stockham_stage<16, 32, ROW_LOG, 2, SEQS, NT>(buf, twl);
+        stockham_stage<16, 32, ROW_LOG, 2, SEQS, NT, true>(buf, ttf);
         stage_last_from_lds<16, N2>(zA, buf + m.sA * SEQS, m.jA, twl);
         stage_last_from_lds<16, N2>(zB, buf + m.sB * SEQS, m.jB, twl);
     };
@@ -340,8 +344,10 @@ __global__ __launch_bounds__(ROWS_NT) void k_rows(float2* __restrict__ Yk, float
     __syncthreads();  // twl
 
     while (true) {
-        auto wkA = [&](int r) { return cmul(wrA, tw_lds(twl, m.jA + 512 * r)); };
-        auto wkB = [&](int r) { return cmul(wrB, tw_lds(twl, m.jB + 512 * r)); };
+        // split twiddles w_L^k at k = row + N1 (j + 512 r): w_L^row w_2^14^j w_32^r
+        const float2 baseA = cmul(wrA, tw_lds(twl, m.jA)), baseB = cmul(wrB, tw_lds(twl, m.jB));
+        auto wkA = [&](int r) { return w32pow(opaque_f2(baseA), r); };
+        auto wkB = [&](int r) { return w32pow(opaque_f2(baseB), r); };
         const uint32_t wn = w + gridDim.x;
         const bool more = wn < items;
         RowMap mn = m;
@@ -443,7 +449,7 @@ __global__ __launch_bounds__(ROWS_NT) void k_rows(float2* __restrict__ Yk, float
         stage_first_to_lds<16>(qA, buf + m.sA * SEQS, m.jA);
         stage_first_to_lds<16>(qB, buf + m.sB * SEQS, m.jB);
         __syncthreads();
-        stockham_stage<16, 16, ROW_LOG, 2, SEQS, NT>(buf, twl);
+        stockham_stage<16, 16, ROW_LOG, 2, SEQS, NT, true>(buf, tti);
         {
             float2 F[32];
             stage_last_from_lds<32, N2>(F, buf + s1seq * SEQS, s1j, twl);
@@ -562,6 +568,8 @@ __global__ __launch_bounds__(inv_cols_threads(LG1)) void k_inv_cols(
         // column t & 1, butterfly t >> 1, rows j + 256 r), stage 3 into registers (thread t:
         // butterfly t of both columns, t1 = t + 512 r), then the epilogue.
         static_assert(W == 2 && NT == 512, "fused column pass expects 2 columns, 512 threads");
+        __shared__ float2 ttf[32 * 17];
+        fill_stage_table<16, 32>(ttf, tw14, NT);
         {
             const int col = threadIdx.x & 1, j = threadIdx.x >> 1;
             const float2* rp = Rp + (size_t)j * g.N2 + grp * W + col;
@@ -572,7 +580,7 @@ __global__ __launch_bounds__(inv_cols_threads(LG1)) void k_inv_cols(
             stage_first_to_lds<32>(v, buf + col * SEQS, j);
         }
         __syncthreads();
-        stockham_stage<16, 32, LG1, 2, SEQS, NT>(buf, twl);
+        stockham_stage<16, 32, LG1, 2, SEQS, NT, true>(buf, ttf);
         float2 G0[16], G1[16];
         stage_last_from_lds<16, N1>(G0, buf, threadIdx.x, twl);
         stage_last_from_lds<16, N1>(G1, buf + SEQS, threadIdx.x, twl);
