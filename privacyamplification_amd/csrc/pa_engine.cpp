@@ -160,6 +160,7 @@ struct pa_engine {
     pa_config cfg{};
     hipStream_t stream = nullptr;
     float2* tw14 = nullptr;
+    float2* cls = nullptr;  // pass-A class tables (pa::CLS_TABLE_TOTAL entries)
     int cus = 256;  // compute units of the device (persistent kernels launch one workgroup each)
     std::map<uint32_t, Twiddles> tw;
     std::map<uint32_t, std::unique_ptr<Workspace>> ws;
@@ -246,6 +247,28 @@ std::vector<float2> twiddles(uint64_t L, uint64_t count, uint64_t stride) {
     return t;
 }
 
+// Pass A's class tables (layout in pa_kernels.hpp): sums of D-th roots of unity over the set bits
+// of a byte, in double precision, rounded once.
+std::vector<float2> class_tables() {
+    std::vector<float2> t(pa::CLS_TABLE_TOTAL);
+    for (int D = 1; D <= 16; D *= 2) {
+        const int BW = pa::cls_bits(D), NB = pa::cls_tables(D);
+        for (int c = 0; c < D; ++c)
+            for (int q = 0; q < NB; ++q)
+                for (int byte = 0; byte < (1 << BW); ++byte) {
+                    double re = 0.0, im = 0.0;
+                    for (int b = 0; b < BW; ++b)
+                        if ((byte >> b) & 1) {
+                            const double a = -2.0 * M_PI * (double)(((BW * q + b) * c) % D) / (double)D;
+                            re += std::cos(a);
+                            im += std::sin(a);
+                        }
+                    t[pa::cls_table_offset(D) + (c * NB + q) * (1 << BW) + byte] = make_float2((float)re, (float)im);
+                }
+    }
+    return t;
+}
+
 uint64_t workspace_bytes(const Geom& g) {
     const uint64_t N = g.N;
     uint64_t b = (16384 + (N > 8192 ? g.L / 16384 : 1)) * sizeof(float2);
@@ -323,7 +346,7 @@ int get_workspace(pa_engine* e, const Geom& g, Workspace** out) {
 // Twiddle tables of transform length L (get_twiddles must have created them).
 pa::Tables tables(pa_engine* e, uint32_t L) {
     const Twiddles& t = e->tw.at(L);
-    return pa::Tables{e->tw14, t.lo, t.hi};
+    return pa::Tables{e->tw14, t.lo, t.hi, e->cls};
 }
 
 hipStream_t stream_of(pa_engine* e, void* s) { return s ? (hipStream_t)s : e->stream; }
@@ -427,6 +450,10 @@ int pa_create(const pa_config* cfg, pa_engine** out) {
     if (rc != PA_OK) return rc;
     auto tw = twiddles(16384, 16384, 1);
     PA_HIP(hipMemcpy(e->tw14, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice));
+    rc = dev_alloc(e.get(), (void**)&e->cls, pa::CLS_TABLE_TOTAL * sizeof(float2));
+    if (rc != PA_OK) return rc;
+    auto cls = class_tables();
+    PA_HIP(hipMemcpy(e->cls, cls.data(), cls.size() * sizeof(float2), hipMemcpyHostToDevice));
     *out = e.release();
     return PA_OK;
 }
@@ -465,6 +492,7 @@ int pa_destroy(pa_engine* e) {
     slot_release(e, &e->batch);
     if (e->split) (void)hipFree(e->split);
     (void)hipFree(e->tw14);
+    (void)hipFree(e->cls);
     (void)hipStreamDestroy(e->stream);
     delete e;
     return PA_OK;

@@ -121,7 +121,7 @@ __global__ __launch_bounds__(256) void k_transpose_bits(const uint32_t* __restri
 // ------------------------------------------------------------------ pass A ----
 // Workgroup (column group grp of 16 columns, class c): computes Y[k1][t2] for k1 = c + D*k1',
 // t2 in the group:  Y = w_N^{t2 k1} * FFT_M( w_N1^{u c} * sum_v z[t2 + N2(u + M v)] w_D^{v c} ).
-// The class sum over v is four 16-entry table lookups per operand half (bits of tb).
+// The class sum over v is one or two 256-entry table lookups per operand half (bits of tb).
 // Plan 16 x 32 for M = 512: stage 1 computed in registers from the bit masks, stage 2 written
 // straight to Y (thread t: column t mod 16, butterfly t / 16, rows k1' = j + 16 r).
 template <int LGM, int D, bool BATCH, int CPW = fwd_cols_width(LGM)>
@@ -130,6 +130,7 @@ __global__ __launch_bounds__(FWD_NT) void k_fwd_cols(const uint32_t* __restrict_
                                                      const float2* __restrict__ tw14,
                                                      const float2* __restrict__ twlo,
                                                      const float2* __restrict__ twhi,
+                                                     const float2* __restrict__ cls,
                                                      const BlockArgs* __restrict__ bt, int is_seed) {
     if constexpr (BATCH) {
         const BlockArgs& a = bt[blockIdx.z];
@@ -144,20 +145,17 @@ __global__ __launch_bounds__(FWD_NT) void k_fwd_cols(const uint32_t* __restrict_
     static_assert(CPW == 16 || LGM != 9, "the fused M = 512 plan needs 16 columns");
     __shared__ float2 buf[CPW * SEQS];
     __shared__ float2 twl[TWL_SIZE];
-    constexpr int NQ = D < 4 ? 1 : D / 4;  // 4-bit tables: v in [4q, 4q+4)
-    __shared__ float2 TQ[NQ][16];         // TQ[q][nib] = sum_b bit_b(nib) w_D^{(4q+b) c}
-    __shared__ float2 TU[M];              // w_N1^{u c}
+    constexpr int BW = cls_bits(D);         // bits per table: v in [BW q, BW q + BW)
+    constexpr int NB = cls_tables(D);       // tables per 16-bit half
+    constexpr uint32_t MASK = (1u << BW) - 1u;
+    __shared__ float2 TB[NB][1 << BW];      // TB[q][byte] = sum_b bit_b(byte) w_D^{(BW q + b) c}
+    __shared__ float2 TU[M];                // w_N1^{u c}
     const uint32_t logical = xcd_logical(blockIdx.x, gridDim.x);
     const uint32_t c = logical % D, grp = logical / D;  // columns [CPW grp, CPW grp + CPW)
     load_twl(twl, tw14, NT);
-    for (int e = threadIdx.x; e < NQ * 16; e += NT) {
-        const int q = e / 16, nib = e % 16;
-        float2 acc = make_float2(0.f, 0.f);
-        for (int b = 0; b < 4; ++b) {
-            const int v = 4 * q + b;
-            if (v < D && ((nib >> b) & 1)) acc = cadd(acc, tw14[((v * c) % D) * (TW14 / D)]);
-        }
-        TQ[q][nib] = acc;
+    {
+        const float2* src = cls + cls_table_offset(D) + (size_t)c * (NB << BW);
+        for (int e = threadIdx.x; e < NB << BW; e += NT) TB[e >> BW][e & (int)MASK] = src[e];
     }
     // w_N1^{u c}: from the 2^14 table while N1 <= 2^14, else from the length-L two-level table
     if (g.N1 <= (uint32_t)TW14) {
@@ -169,11 +167,11 @@ __global__ __launch_bounds__(FWD_NT) void k_fwd_cols(const uint32_t* __restrict_
     }
     // the element z(col, u) of this class (both halves of tb: even and odd bits)
     auto zval = [&](uint32_t t, int u) {
-        float2 sE = TQ[0][t & 15u], sO = TQ[0][(t >> 16) & 15u];
+        float2 sE = TB[0][t & MASK], sO = TB[0][(t >> 16) & MASK];
 #pragma unroll
-        for (int q = 1; q < NQ; ++q) {
-            sE = cadd(sE, TQ[q][(t >> (4 * q)) & 15u]);
-            sO = cadd(sO, TQ[q][(t >> (16 + 4 * q)) & 15u]);
+        for (int q = 1; q < NB; ++q) {
+            sE = cadd(sE, TB[q][(t >> (BW * q)) & MASK]);
+            sO = cadd(sO, TB[q][(t >> (16 + BW * q)) & MASK]);
         }
         float2 z = make_float2(sE.x - sO.y, sE.y + sO.x);
         if (c == 0) {
@@ -914,7 +912,7 @@ namespace {
 template <int LGM, int D>
 hipError_t launch_fwd_cols_t(const uint32_t* tb, float2* Y, const Geom& g, const Tables& t, hipStream_t s) {
     const uint32_t grid = (g.N2 / (uint32_t)fwd_cols_width(LGM)) * (uint32_t)D;
-    k_fwd_cols<LGM, D, false><<<grid, FWD_NT, 0, s>>>(tb, Y, g, t.tw14, t.lo, t.hi, nullptr, 0);
+    k_fwd_cols<LGM, D, false><<<grid, FWD_NT, 0, s>>>(tb, Y, g, t.tw14, t.lo, t.hi, t.cls, nullptr, 0);
     return hipGetLastError();
 }
 template <int LG1>
@@ -1072,7 +1070,7 @@ template <int LGM, int D>
 hipError_t batch_fwd_cols_t(const BlockArgs* bt, uint32_t count, const Geom& g0, const Tables& t, int is_seed,
                             hipStream_t s) {
     k_fwd_cols<LGM, D, true><<<dim3((g0.N2 / (uint32_t)fwd_cols_width(LGM)) * (uint32_t)D, 1, count), FWD_NT, 0, s>>>(
-        nullptr, nullptr, g0, t.tw14, t.lo, t.hi, bt, is_seed);
+        nullptr, nullptr, g0, t.tw14, t.lo, t.hi, t.cls, bt, is_seed);
     return hipGetLastError();
 }
 template <int LG1>

@@ -17,11 +17,22 @@ constexpr int PACK_G = 256;
 
 // Device twiddle tables: tw14[x] = exp(-2 pi i x / 2^14); lo/hi: exp(-2 pi i x / L) split as
 // lo[x mod 2^14] * hi[x >> 14].
+// cls: pass A's class tables, cls_table_offset(D) onwards: entry [(c NB + q) 2^BW + byte] =
+// sum over the set bits b of byte of exp(-2 pi i ((BW q + b) c mod D) / D), BW = min(D, 8),
+// NB = D / BW; computed in double precision and rounded once.
 struct Tables {
     const float2* tw14;
     const float2* lo;
     const float2* hi;
+    const float2* cls;
 };
+__host__ __device__ constexpr int cls_bits(int D) { return D < 8 ? D : 8; }
+__host__ __device__ constexpr int cls_tables(int D) { return D / cls_bits(D); }
+__host__ __device__ constexpr int cls_table_size(int D) { return D * cls_tables(D) * (1 << cls_bits(D)); }
+__host__ __device__ constexpr int cls_table_offset(int D) {
+    return D <= 1 ? 0 : cls_table_offset(D / 2) + cls_table_size(D / 2);
+}
+constexpr int CLS_TABLE_TOTAL = cls_table_offset(32);  // D = 1 .. 16
 
 // Columns per workgroup in pass A' (tile of W x N1 complex <= 16384 values) and its threads.
 __host__ __device__ constexpr int inv_cols_width(int lg1) {
