@@ -511,7 +511,8 @@ __global__ __launch_bounds__(inv_cols_threads(LG1)) void k_inv_cols(
     constexpr int NT = inv_cols_threads(LG1);
     constexpr int LGL = LG1 < 14 ? LG1 : 14;  // LDS transform length (N1 = 2^15: two halves)
     constexpr int SEQS = seq_stride(1 << LGL);
-    __shared__ float2 buf[W * SEQS];
+    constexpr int WL = LG1 == 13 ? 2 : W;      // columns held in LDS at once
+    __shared__ float2 buf[WL * SEQS];
     __shared__ float2 twl[TWL_SIZE];
     load_twl(twl, tw14, NT);
     const uint32_t grp = xcd_logical(blockIdx.x, gridDim.x);
@@ -562,40 +563,49 @@ __global__ __launch_bounds__(inv_cols_threads(LG1)) void k_inv_cols(
         return;
     } else {
     if constexpr (LG1 == 13) {
-        // plan 32 x 16 x 16 over the two columns: stage 1 straight from global memory (thread t:
-        // column t & 1, butterfly t >> 1, rows j + 256 r), stage 3 into registers (thread t:
-        // butterfly t of both columns, t1 = t + 512 r), then the epilogue.
-        static_assert(W == 2 && NT == 512, "fused column pass expects 2 columns, 512 threads");
+        // Four columns t2 = 4 grp + 2 p + col as two pairs p through a two-column buffer.  Plan
+        // 32 x 16 x 16 per pair: stage 1 straight from global memory (thread t: column t & 1,
+        // butterfly t >> 1, rows j + 256 r), stage 3 into registers (thread t: butterfly t of both
+        // columns, t1 = t + 512 r), then the epilogue.  Both pairs' loads are issued first, so the
+        // second pair arrives while the first is transformed, and each 128-byte row segment serves
+        // 32 bytes to this workgroup.
+        static_assert(W == 4 && NT == 512, "fused column pass expects 4 columns, 512 threads");
         __shared__ float2 ttf[32 * 17];
         fill_stage_table<16, 32>(ttf, tw14, NT);
+        const int col = threadIdx.x & 1, j = threadIdx.x >> 1;
+        float2 va[32], vb[32];
         {
-            const int col = threadIdx.x & 1, j = threadIdx.x >> 1;
             const float2* rp = Rp + (size_t)j * g.N2 + grp * W + col;
-            float2 v[32];
 #pragma unroll
-            for (int r = 0; r < 32; ++r) v[r] = rp[(size_t)256 * r * g.N2];
-            __syncthreads();  // twl
-            stage_first_to_lds<32>(v, buf + col * SEQS, j);
+            for (int r = 0; r < 32; ++r) va[r] = rp[(size_t)256 * r * g.N2];
+#pragma unroll
+            for (int r = 0; r < 32; ++r) vb[r] = rp[(size_t)256 * r * g.N2 + 2];
         }
-        __syncthreads();
-        stockham_stage<16, 32, LG1, 2, SEQS, NT, true>(buf, ttf);
-        float2 G0[16], G1[16];
-        stage_last_from_lds<16, N1>(G0, buf, threadIdx.x, twl);
-        stage_last_from_lds<16, N1>(G1, buf + SEQS, threadIdx.x, twl);
         const float corr = correction(pop_total(wk_cnt), pop_total(ws_cnt), g.L);
         const float invN = g.inv_N;
         float dist = 0.f;
-        const uint32_t t2 = grp * W;
+        __syncthreads();  // twl, ttf
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const uint32_t t1 = threadIdx.x + 512u * r;
-            const uint32_t i0 = 2u * (t2 + g.N2 * t1);
-            if (i0 + 4u <= g.out_off || i0 >= g.out_off + g.out_bits) continue;
-            const uint32_t b0 = round_bit(G0[r].x * invN + corr, dist);
-            const uint32_t b1 = round_bit(-G0[r].y * invN + corr, dist);
-            const uint32_t b2 = round_bit(G1[r].x * invN + corr, dist);
-            const uint32_t b3 = round_bit(-G1[r].y * invN + corr, dist);
-            nibT[(size_t)grp * N1 + t1] = (uint8_t)((b0 << 3) | (b1 << 2) | (b2 << 1) | b3);
+        for (int p = 0; p < 2; ++p) {
+            if (p == 1) __syncthreads();  // the first pair's stage-3 reads are done
+            stage_first_to_lds<32>(p == 0 ? va : vb, buf + col * SEQS, j);
+            __syncthreads();
+            stockham_stage<16, 32, LG1, 2, SEQS, NT, true>(buf, ttf);
+            float2 G0[16], G1[16];
+            stage_last_from_lds<16, N1>(G0, buf, threadIdx.x, twl);
+            stage_last_from_lds<16, N1>(G1, buf + SEQS, threadIdx.x, twl);
+            const uint32_t t2 = grp * W + 2u * (uint32_t)p;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const uint32_t t1 = threadIdx.x + 512u * r;
+                const uint32_t i0 = 2u * (t2 + g.N2 * t1);
+                if (i0 + 4u <= g.out_off || i0 >= g.out_off + g.out_bits) continue;
+                const uint32_t b0 = round_bit(G0[r].x * invN + corr, dist);
+                const uint32_t b1 = round_bit(-G0[r].y * invN + corr, dist);
+                const uint32_t b2 = round_bit(G1[r].x * invN + corr, dist);
+                const uint32_t b3 = round_bit(-G1[r].y * invN + corr, dist);
+                nibT[(size_t)(t2 >> 1) * N1 + t1] = (uint8_t)((b0 << 3) | (b1 << 2) | (b2 << 1) | b3);
+            }
         }
         if (margin) margin_publish(dist, margin);
         return;

@@ -34,9 +34,10 @@ __host__ __device__ constexpr int cls_table_offset(int D) {
 }
 constexpr int CLS_TABLE_TOTAL = cls_table_offset(32);  // D = 1 .. 16
 
-// Columns per workgroup in pass A' (tile of W x N1 complex <= 16384 values) and its threads.
+// Columns per workgroup in pass A' (tile of W x N1 complex <= 16384 values in LDS; N1 = 8192 takes
+// 4 columns as two pairs through a 2-column buffer) and its threads.
 __host__ __device__ constexpr int inv_cols_width(int lg1) {
-    return lg1 >= 14 ? 1 : ((14 - lg1) >= 4 ? 16 : (1 << (14 - lg1)));
+    return lg1 >= 14 ? 1 : (lg1 == 13 ? 4 : ((14 - lg1) >= 4 ? 16 : (1 << (14 - lg1))));
 }
 __host__ __device__ constexpr int clamp_threads(int t) { return t < 64 ? 64 : (t > 1024 ? 1024 : t); }
 __host__ __device__ constexpr int inv_cols_threads(int lg1) {
