@@ -289,23 +289,27 @@ __device__ __forceinline__ float2 tw_two_level(const float2* __restrict__ lo,
 }
 
 // Real-FFT split: given Z = FFT_N(x_even + i x_odd) at k and Z at (N-k) mod N, and w = exp(-2 pi i k/(2N)),
-// returns X[k] of the length-2N real transform.
+// returns 2 X[k] of the length-2N real transform (the factor 1/2 of the textbook split is left out
+// here and in c2r_split; the epilogue's normalisation 1/(8N) absorbs the exact factor 8 that the
+// two forward splits and the inverse split leave on the result).
 __device__ __forceinline__ float2 r2c_split(float2 zk, float2 zp, float2 w) {
     const float2 zc = conjf2(zp);
-    const float2 e = cscale(cadd(zk, zc), 0.5f);
-    const float2 d = cscale(csub(zk, zc), 0.5f);
+    const float2 e = cadd(zk, zc);
+    const float2 d = csub(zk, zc);
     const float2 t = cmul(w, d);
     return make_float2(e.x + t.y, e.y - t.x);  // e - i*t
 }
 // Inverse split: from P[k], P[N-k] of a Hermitian length-2N spectrum, returns
-// Q[k] = FFT_N(c_even + i c_odd)[k].
+// 2 Q[k], Q[k] = FFT_N(c_even + i c_odd)[k].
 __device__ __forceinline__ float2 c2r_split(float2 pk, float2 pp, float2 w) {
     const float2 pc = conjf2(pp);
-    const float2 e = cscale(cadd(pk, pc), 0.5f);
-    const float2 d = cscale(csub(pk, pc), 0.5f);
+    const float2 e = cadd(pk, pc);
+    const float2 d = csub(pk, pc);
     const float2 t = cmul(conjf2(w), d);
     return make_float2(e.x - t.y, e.y + t.x);  // e + i*t
 }
+// 2 X[N] (Nyquist) of the length-2N real transform from Z[0], matching r2c_split's scale.
+__device__ __forceinline__ float2 r2c_nyquist(float2 z0) { return make_float2(2.f * (z0.x - z0.y), 0.f); }
 
 // Wave-level max of a non-negative float published to a device float with an atomic max
 // on its bit pattern (non-negative IEEE floats order like their unsigned bit patterns).

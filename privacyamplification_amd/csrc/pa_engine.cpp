@@ -93,7 +93,7 @@ int make_geom(uint32_t n, uint32_t m, uint32_t flags, uint32_t reverse_endian, u
     x.key_hw = x.h / 32;
     x.do_xor = (flags & PA_XOR_KEY_REST) ? 1u : 0u;
     x.reverse_endian = reverse_endian ? 1u : 0u;
-    x.inv_N = 1.0f / (float)x.N;
+    x.inv_N = 1.0f / (8.0f * (float)x.N);  // exact: includes the factor 8 the real splits leave
     *g = x;
     return PA_OK;
 }

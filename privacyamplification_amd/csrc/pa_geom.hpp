@@ -33,7 +33,7 @@ struct Geom {
     uint32_t seed_words, seed_rot;     // n/32, rotation of the seed in words (m/32 when L > n)
     uint32_t key_hw;                   // h/32: index of the key word masked to its MSB
     uint32_t do_xor, reverse_endian;
-    float inv_N;
+    float inv_N;                       // 1/(8N): output normalisation (r2c_split / c2r_split scale)
 };
 
 // One block of a batched launch: its geometry and device buffers.  The blocks of one launch share

@@ -73,14 +73,17 @@ __global__ __launch_bounds__(256) void k_zero_u32(uint32_t* p, uint32_t count, c
 
 // ------------------------------------------------------------------ bit transpose ----
 // tb[u*N2 + t2] = E | O << 16 with E bit v = x[2(t2 + N2(u + M v))], O bit v = the odd bit.
-// Each thread writes TB_PER outputs; popcounts are reduced per workgroup into one of POP_SLOTS
-// counters.
+// Workgroup (u, part): thread k covers t2 in [part*TB_SPAN + 32k, +32): for each class v it reads
+// the two words holding those 64 bits (coalesced across the workgroup), transposes the D x 32
+// bit-pairs in registers, and the workgroup writes its TB_SPAN outputs through LDS.  Popcounts
+// are reduced per workgroup into one of POP_SLOTS counters.
 template <int D, bool BATCH>
-__global__ __launch_bounds__(256) void k_transpose_bits(const uint32_t* __restrict__ bits,
-                                                        uint32_t* __restrict__ tb,
-                                                        uint32_t* __restrict__ count, Geom g,
-                                                        int is_seed, const BlockArgs* __restrict__ bt) {
-    __shared__ uint32_t red[4];
+__global__ __launch_bounds__(TB_NT) void k_transpose_bits(const uint32_t* __restrict__ bits,
+                                                          uint32_t* __restrict__ tb,
+                                                          uint32_t* __restrict__ count, Geom g,
+                                                          int is_seed, const BlockArgs* __restrict__ bt) {
+    __shared__ uint32_t stage[TB_NT * 33];
+    __shared__ uint32_t red[TB_NT / 64];
     if constexpr (BATCH) {
         const BlockArgs& a = bt[blockIdx.z];
         if (is_seed && a.seed == nullptr) return;  // resident seed: nothing to transpose
@@ -89,31 +92,40 @@ __global__ __launch_bounds__(256) void k_transpose_bits(const uint32_t* __restri
         tb = is_seed ? a.tbs : a.tbk;
         count = a.counts + (is_seed ? POP_SLOTS : 0);
     }
-    const uint32_t total = g.M * g.N2;
+    constexpr int PARTS_LOG = ROW_LOG - TB_SPAN_LOG;
+    const uint32_t u = blockIdx.x >> PARTS_LOG, part = blockIdx.x & ((1u << PARTS_LOG) - 1u);
+    const uint32_t t0 = (part << TB_SPAN_LOG) + 32u * threadIdx.x;
+    uint32_t w0[D], w1[D];
     uint32_t pc = 0;
 #pragma unroll
-    for (int q = 0; q < TB_PER; ++q) {
-        const uint32_t idx = (blockIdx.x * TB_PER + q) * 256u + threadIdx.x;
-        if (idx < total) {
-            const uint32_t t2 = idx & (g.N2 - 1u), u = idx >> g.lg2;
-            uint32_t E = 0, O = 0;
+    for (int v = 0; v < D; ++v) {
+        const uint32_t widx = (2u * (t0 + g.N2 * (u + g.M * (uint32_t)v))) >> 5;  // even
+        w0[v] = is_seed ? seed_word(bits, g, widx) : key_word(bits, g, widx);
+        w1[v] = is_seed ? seed_word(bits, g, widx + 1u) : key_word(bits, g, widx + 1u);
+        pc += (uint32_t)(__popc(w0[v]) + __popc(w1[v]));
+    }
+    uint32_t* srow = stage + threadIdx.x * 33;
 #pragma unroll
-            for (int v = 0; v < D; ++v) {
-                const uint32_t pos = 2u * (t2 + g.N2 * (u + g.M * (uint32_t)v));
-                const uint32_t widx = pos >> 5, sh = pos & 31u;
-                const uint32_t w = is_seed ? seed_word(bits, g, widx) : key_word(bits, g, widx);
-                E |= ((w >> (31u - sh)) & 1u) << v;
-                O |= ((w >> (30u - sh)) & 1u) << v;
-            }
-            tb[idx] = E | (O << 16);
-            pc += (uint32_t)(__popc(E) + __popc(O));
+    for (int i = 0; i < 32; ++i) {
+        const int sh = 31 - 2 * (i & 15);  // even bit 2i of the 64-bit pair, MSB first
+        uint32_t E = 0, O = 0;
+#pragma unroll
+        for (int v = 0; v < D; ++v) {
+            const uint32_t w = i < 16 ? w0[v] : w1[v];
+            E |= ((w >> sh) & 1u) << v;
+            O |= ((w >> (sh - 1)) & 1u) << v;
         }
+        srow[i] = E | (O << 16);
     }
     for (int off = 32; off > 0; off >>= 1) pc += __shfl_xor(pc, off);
     if ((threadIdx.x & 63u) == 0u) red[threadIdx.x >> 6] = pc;
     __syncthreads();
+    uint32_t* out = tb + (size_t)u * g.N2 + (part << TB_SPAN_LOG);
+    for (int e = threadIdx.x; e < (1 << TB_SPAN_LOG); e += TB_NT) out[e] = stage[(e >> 5) * 33 + (e & 31)];
     if (threadIdx.x == 0) {
-        const uint32_t t = red[0] + red[1] + red[2] + red[3];
+        uint32_t t = 0;
+#pragma unroll
+        for (int k = 0; k < TB_NT / 64; ++k) t += red[k];
         if (t) atomicAdd(count + (blockIdx.x % POP_SLOTS), t);
     }
 }
@@ -367,7 +379,7 @@ __global__ __launch_bounds__(ROWS_NT) void k_rows(float2* __restrict__ Yk, float
                     SB[15 - r] = r2c_split(zB[15 - r], zA[r], wkB(15 - r));
                 }
             } else {
-                Sn = make_float2(zA[0].x - zA[0].y, 0.f);
+                Sn = r2c_nyquist(zA[0]);
                 SA[0] = make_float2(0.f, 0.f);
                 SA[8] = r2c_split(zA[8], zA[8], wkA(8));
 #pragma unroll
@@ -428,7 +440,7 @@ __global__ __launch_bounds__(ROWS_NT) void k_rows(float2* __restrict__ Yk, float
                 for (int r = 0; r < 16; ++r)
                     unit(zA[r], zB[15 - r], SA[r], SB[15 - r], wkA(r), wkB(15 - r), qA[r], qB[15 - r]);
             } else {
-                const float2 pn = cmul(make_float2(zA[0].x - zA[0].y, 0.f), Sn);
+                const float2 pn = cmul(r2c_nyquist(zA[0]), Sn);
                 qA[0] = conjf2(c2r_split(make_float2(0.f, 0.f), pn, make_float2(1.f, 0.f)));
                 {
                     const float2 w8 = wkA(8);
@@ -778,7 +790,7 @@ __global__ __launch_bounds__(small_threads(LGN)) void k_small(
             const float2 zk = buf[SEQS + padi(k)], zp = buf[SEQS + padi(kp)];
             s = r2c_split(zk, zp, wk);
             if (k == 0) {
-                sn = make_float2(zk.x - zk.y, 0.f);
+                sn = r2c_nyquist(zk);
                 s = make_float2(0.f, 0.f);
             }
         } else {
@@ -793,7 +805,7 @@ __global__ __launch_bounds__(small_threads(LGN)) void k_small(
         const float2 zk = buf[padi(k)], zp = buf[padi(kp)];
         float2 x = r2c_split(zk, zp, wk);
         if (k == 0) {
-            PN = cmul(make_float2(zk.x - zk.y, 0.f), sn);
+            PN = cmul(r2c_nyquist(zk), sn);
             x = make_float2(0.f, 0.f);
         }
         X[i] = cmul(x, s);
@@ -935,9 +947,8 @@ hipError_t launch_inv_cols_t(const float2* R, uint8_t* nib, const Geom& g, const
 template <int D>
 hipError_t launch_transpose_t(const uint32_t* bits, uint32_t* tb, uint32_t* count, const Geom& g,
                               int is_seed, hipStream_t s) {
-    const uint32_t total = g.M * g.N2;
-    const uint32_t per_block = 256u * TB_PER;
-    k_transpose_bits<D, false><<<(total + per_block - 1u) / per_block, 256, 0, s>>>(bits, tb, count, g, is_seed, nullptr);
+    const uint32_t grid = g.M << (ROW_LOG - TB_SPAN_LOG);
+    k_transpose_bits<D, false><<<grid, TB_NT, 0, s>>>(bits, tb, count, g, is_seed, nullptr);
     return hipGetLastError();
 }
 template <int LGN>
@@ -1071,8 +1082,7 @@ hipError_t launch_batch_zero(const BlockArgs* bt, uint32_t count, hipStream_t s)
 namespace {
 template <int D>
 hipError_t batch_transpose_t(const BlockArgs* bt, uint32_t count, const Geom& g0, int is_seed, hipStream_t s) {
-    const uint32_t total = g0.M * g0.N2, per_block = 256u * TB_PER;
-    k_transpose_bits<D, true><<<dim3((total + per_block - 1u) / per_block, 1, count), 256, 0, s>>>(
+    k_transpose_bits<D, true><<<dim3(g0.M << (ROW_LOG - TB_SPAN_LOG), 1, count), TB_NT, 0, s>>>(
         nullptr, nullptr, nullptr, g0, is_seed, bt);
     return hipGetLastError();
 }

@@ -10,7 +10,8 @@ namespace pa {
 enum { ROWS_DYN = 0, ROWS_STATIC = 1, ROWS_SEEDPREP = 2 };
 constexpr int ROWS_NT = 512;  // threads of the row pass (256-VGPR budget per lane)
 constexpr int FWD_NT = 256;   // threads of the forward column pass
-constexpr int TB_PER = 4;     // bit-transpose outputs per thread
+constexpr int TB_NT = 128;        // bit-transpose threads: 32 outputs each
+constexpr int TB_SPAN_LOG = 12;   // bit-transpose outputs per workgroup (2^12 of a row of N2 = 2^13)
 constexpr int POP_SLOTS = 64; // partial popcount counters per operand
 constexpr int PACK_T1 = 32;   // output packer tile: rows x column pairs
 constexpr int PACK_G = 256;
