@@ -39,9 +39,9 @@ def kernel_bytes(n, m):
     """Algorithmic bytes per launch of each kernel as built."""
     return {
         "k_transpose_bits(key)": n / 8 + n / 8,
-        "k_transpose_bits(seed)": n / 8 + n / 8,
+        "k_transpose_bits(key+seed)": 2 * (n / 8 + n / 8),
         "k_fwd_cols(key)": n / 8 + 4 * n,
-        "k_fwd_cols(seed)": n / 8 + 4 * n,
+        "k_fwd_cols(key+seed)": 2 * (n / 8 + 4 * n),
         "k_rows(seedprep)": 8 * n,
         "k_rows": 12 * n,
         "k_inv_cols": 4 * n + m / 4,

@@ -351,11 +351,11 @@ pa::Tables tables(pa_engine* e, uint32_t L) {
 
 hipStream_t stream_of(pa_engine* e, void* s) { return s ? (hipStream_t)s : e->stream; }
 
-enum Stage { ST_SMALL, ST_ZERO, ST_TRANSPOSE_KEY, ST_FWD_COLS_KEY, ST_TRANSPOSE_SEED, ST_FWD_COLS_SEED,
+enum Stage { ST_SMALL, ST_ZERO, ST_TRANSPOSE_KEY, ST_FWD_COLS_KEY, ST_TRANSPOSE_BOTH, ST_FWD_COLS_BOTH,
              ST_ROWS_SEEDPREP, ST_ROWS, ST_INV_COLS, ST_PACK, ST_COUNT };
 const char* const kStageNames[ST_COUNT] = {
-    "k_small", "k_zero_u32", "k_transpose_bits(key)", "k_fwd_cols(key)", "k_transpose_bits(seed)",
-    "k_fwd_cols(seed)", "k_rows(seedprep)", "k_rows", "k_inv_cols", "k_pack"};
+    "k_small", "k_zero_u32", "k_transpose_bits(key)", "k_fwd_cols(key)", "k_transpose_bits(key+seed)",
+    "k_fwd_cols(key+seed)", "k_rows(seedprep)", "k_rows", "k_inv_cols", "k_pack"};
 
 hipEvent_t pool_event(pa_engine* e) {
     if (!e->event_pool.empty()) {
@@ -396,15 +396,17 @@ int run_device(pa_engine* e, const Geom& g, Workspace* w, const uint32_t* key, c
     uint32_t* key_cnt = w->counts;
     uint32_t* seed_cnt = w->counts + pa::POP_SLOTS;
     PA_HIP(timed(e, ST_ZERO, s, [&] { return pa::launch_zero_u32(w->counts, 2 * pa::POP_SLOTS, s); }));
-    PA_HIP(timed(e, ST_TRANSPOSE_KEY, s, [&] { return pa::launch_transpose(key, w->tbk, key_cnt, g, 0, s); }));
-    PA_HIP(timed(e, ST_FWD_COLS_KEY, s, [&] { return pa::launch_fwd_cols(w->tbk, w->Yk, g, t, s); }));
-    if (!sspec) {
-        PA_HIP(timed(e, ST_TRANSPOSE_SEED, s, [&] { return pa::launch_transpose(seed, w->tbs, seed_cnt, g, 1, s); }));
-        PA_HIP(timed(e, ST_FWD_COLS_SEED, s, [&] { return pa::launch_fwd_cols(w->tbs, w->Ys, g, t, s); }));
+    if (!sspec) {  // key and seed share each launch of the bit transpose and of pass A
+        PA_HIP(timed(e, ST_TRANSPOSE_BOTH, s, [&] {
+            return pa::launch_transpose(key, w->tbk, key_cnt, g, 0, s, seed, w->tbs);
+        }));
+        PA_HIP(timed(e, ST_FWD_COLS_BOTH, s, [&] { return pa::launch_fwd_cols(w->tbk, w->Yk, g, t, s, w->tbs, w->Ys); }));
         PA_HIP(timed(e, ST_ROWS, s, [&] { return pa::launch_rows(pa::ROWS_DYN, w->Yk, w->Ys, g, t, e->cus, s); }));
         ws_dev = seed_cnt;
     } else {
         float2* S = const_cast<float2*>(sspec);
+        PA_HIP(timed(e, ST_TRANSPOSE_KEY, s, [&] { return pa::launch_transpose(key, w->tbk, key_cnt, g, 0, s); }));
+        PA_HIP(timed(e, ST_FWD_COLS_KEY, s, [&] { return pa::launch_fwd_cols(w->tbk, w->Yk, g, t, s); }));
         PA_HIP(timed(e, ST_ROWS, s, [&] { return pa::launch_rows(pa::ROWS_STATIC, w->Yk, S, g, t, e->cus, s); }));
     }
     PA_HIP(timed(e, ST_INV_COLS, s, [&] { return pa::launch_inv_cols(w->Yk, w->nib, g, key_cnt, ws_dev, t, margin, s); }));

@@ -81,9 +81,17 @@ template <int D, bool BATCH>
 __global__ __launch_bounds__(TB_NT) void k_transpose_bits(const uint32_t* __restrict__ bits,
                                                           uint32_t* __restrict__ tb,
                                                           uint32_t* __restrict__ count, Geom g,
-                                                          int is_seed, const BlockArgs* __restrict__ bt) {
+                                                          int is_seed, const BlockArgs* __restrict__ bt,
+                                                          const uint32_t* __restrict__ bits2,
+                                                          uint32_t* __restrict__ tb2) {
     __shared__ uint32_t stage[TB_NT * 33];
     __shared__ uint32_t red[TB_NT / 64];
+    if (!BATCH && blockIdx.y == 1) {  // key and seed in one launch: grid.y selects the seed
+        bits = bits2;
+        tb = tb2;
+        count += POP_SLOTS;
+        is_seed = 1;
+    }
     if constexpr (BATCH) {
         const BlockArgs& a = bt[blockIdx.z];
         if (is_seed && a.seed == nullptr) return;  // resident seed: nothing to transpose
@@ -143,7 +151,13 @@ __global__ __launch_bounds__(FWD_NT) void k_fwd_cols(const uint32_t* __restrict_
                                                      const float2* __restrict__ twlo,
                                                      const float2* __restrict__ twhi,
                                                      const float2* __restrict__ cls,
-                                                     const BlockArgs* __restrict__ bt, int is_seed) {
+                                                     const BlockArgs* __restrict__ bt, int is_seed,
+                                                     const uint32_t* __restrict__ tb2,
+                                                     float2* __restrict__ Y2) {
+    if (!BATCH && blockIdx.y == 1) {  // key and seed in one launch: grid.y selects the seed
+        tb = tb2;
+        Y = Y2;
+    }
     if constexpr (BATCH) {
         const BlockArgs& a = bt[blockIdx.z];
         if (is_seed && a.seed == nullptr) return;
@@ -932,9 +946,11 @@ hipError_t launch_split_finish(const uint32_t* acc, const uint32_t* key, uint32_
 // ================================================================== launchers ====
 namespace {
 template <int LGM, int D>
-hipError_t launch_fwd_cols_t(const uint32_t* tb, float2* Y, const Geom& g, const Tables& t, hipStream_t s) {
+hipError_t launch_fwd_cols_t(const uint32_t* tb, float2* Y, const Geom& g, const Tables& t, hipStream_t s,
+                             const uint32_t* tb2, float2* Y2) {
     const uint32_t grid = (g.N2 / (uint32_t)fwd_cols_width(LGM)) * (uint32_t)D;
-    k_fwd_cols<LGM, D, false><<<grid, FWD_NT, 0, s>>>(tb, Y, g, t.tw14, t.lo, t.hi, t.cls, nullptr, 0);
+    k_fwd_cols<LGM, D, false><<<dim3(grid, Y2 ? 2 : 1), FWD_NT, 0, s>>>(tb, Y, g, t.tw14, t.lo, t.hi, t.cls, nullptr, 0,
+                                                                    tb2, Y2);
     return hipGetLastError();
 }
 template <int LG1>
@@ -946,9 +962,9 @@ hipError_t launch_inv_cols_t(const float2* R, uint8_t* nib, const Geom& g, const
 }
 template <int D>
 hipError_t launch_transpose_t(const uint32_t* bits, uint32_t* tb, uint32_t* count, const Geom& g,
-                              int is_seed, hipStream_t s) {
-    const uint32_t grid = g.M << (ROW_LOG - TB_SPAN_LOG);
-    k_transpose_bits<D, false><<<grid, TB_NT, 0, s>>>(bits, tb, count, g, is_seed, nullptr);
+                              int is_seed, hipStream_t s, const uint32_t* bits2, uint32_t* tb2) {
+    const dim3 grid(g.M << (ROW_LOG - TB_SPAN_LOG), bits2 ? 2 : 1);
+    k_transpose_bits<D, false><<<grid, TB_NT, 0, s>>>(bits, tb, count, g, is_seed, nullptr, bits2, tb2);
     return hipGetLastError();
 }
 template <int LGN>
@@ -971,34 +987,35 @@ hipError_t launch_zero_u32(uint32_t* p, uint32_t count, hipStream_t s) {
 }
 
 hipError_t launch_transpose(const uint32_t* bits, uint32_t* tb, uint32_t* count, const Geom& g,
-                            int is_seed, hipStream_t s) {
+                            int is_seed, hipStream_t s, const uint32_t* bits2, uint32_t* tb2) {
     switch (g.D) {
-    case 1: return launch_transpose_t<1>(bits, tb, count, g, is_seed, s);
-    case 2: return launch_transpose_t<2>(bits, tb, count, g, is_seed, s);
-    case 4: return launch_transpose_t<4>(bits, tb, count, g, is_seed, s);
-    case 8: return launch_transpose_t<8>(bits, tb, count, g, is_seed, s);
-    case 16: return launch_transpose_t<16>(bits, tb, count, g, is_seed, s);
+    case 1: return launch_transpose_t<1>(bits, tb, count, g, is_seed, s, bits2, tb2);
+    case 2: return launch_transpose_t<2>(bits, tb, count, g, is_seed, s, bits2, tb2);
+    case 4: return launch_transpose_t<4>(bits, tb, count, g, is_seed, s, bits2, tb2);
+    case 8: return launch_transpose_t<8>(bits, tb, count, g, is_seed, s, bits2, tb2);
+    case 16: return launch_transpose_t<16>(bits, tb, count, g, is_seed, s, bits2, tb2);
     default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_fwd_cols(const uint32_t* tb, float2* Y, const Geom& g, const Tables& t, hipStream_t s) {
+hipError_t launch_fwd_cols(const uint32_t* tb, float2* Y, const Geom& g, const Tables& t, hipStream_t s,
+                           const uint32_t* tb2, float2* Y2) {
     switch (g.lg1) {
-    case 1: return launch_fwd_cols_t<1, 1>(tb, Y, g, t, s);
-    case 2: return launch_fwd_cols_t<2, 1>(tb, Y, g, t, s);
-    case 3: return launch_fwd_cols_t<3, 1>(tb, Y, g, t, s);
-    case 4: return launch_fwd_cols_t<4, 1>(tb, Y, g, t, s);
-    case 5: return launch_fwd_cols_t<5, 1>(tb, Y, g, t, s);
-    case 6: return launch_fwd_cols_t<6, 1>(tb, Y, g, t, s);
-    case 7: return launch_fwd_cols_t<7, 1>(tb, Y, g, t, s);
-    case 8: return launch_fwd_cols_t<8, 1>(tb, Y, g, t, s);
-    case 9: return launch_fwd_cols_t<9, 1>(tb, Y, g, t, s);
-    case 10: return launch_fwd_cols_t<9, 2>(tb, Y, g, t, s);
-    case 11: return launch_fwd_cols_t<9, 4>(tb, Y, g, t, s);
-    case 12: return launch_fwd_cols_t<9, 8>(tb, Y, g, t, s);
-    case 13: return launch_fwd_cols_t<9, 16>(tb, Y, g, t, s);
-    case 14: return launch_fwd_cols_t<10, 16>(tb, Y, g, t, s);
-    case 15: return launch_fwd_cols_t<11, 16>(tb, Y, g, t, s);
+    case 1: return launch_fwd_cols_t<1, 1>(tb, Y, g, t, s, tb2, Y2);
+    case 2: return launch_fwd_cols_t<2, 1>(tb, Y, g, t, s, tb2, Y2);
+    case 3: return launch_fwd_cols_t<3, 1>(tb, Y, g, t, s, tb2, Y2);
+    case 4: return launch_fwd_cols_t<4, 1>(tb, Y, g, t, s, tb2, Y2);
+    case 5: return launch_fwd_cols_t<5, 1>(tb, Y, g, t, s, tb2, Y2);
+    case 6: return launch_fwd_cols_t<6, 1>(tb, Y, g, t, s, tb2, Y2);
+    case 7: return launch_fwd_cols_t<7, 1>(tb, Y, g, t, s, tb2, Y2);
+    case 8: return launch_fwd_cols_t<8, 1>(tb, Y, g, t, s, tb2, Y2);
+    case 9: return launch_fwd_cols_t<9, 1>(tb, Y, g, t, s, tb2, Y2);
+    case 10: return launch_fwd_cols_t<9, 2>(tb, Y, g, t, s, tb2, Y2);
+    case 11: return launch_fwd_cols_t<9, 4>(tb, Y, g, t, s, tb2, Y2);
+    case 12: return launch_fwd_cols_t<9, 8>(tb, Y, g, t, s, tb2, Y2);
+    case 13: return launch_fwd_cols_t<9, 16>(tb, Y, g, t, s, tb2, Y2);
+    case 14: return launch_fwd_cols_t<10, 16>(tb, Y, g, t, s, tb2, Y2);
+    case 15: return launch_fwd_cols_t<11, 16>(tb, Y, g, t, s, tb2, Y2);
     default: return hipErrorInvalidValue;
     }
 }
@@ -1083,14 +1100,14 @@ namespace {
 template <int D>
 hipError_t batch_transpose_t(const BlockArgs* bt, uint32_t count, const Geom& g0, int is_seed, hipStream_t s) {
     k_transpose_bits<D, true><<<dim3(g0.M << (ROW_LOG - TB_SPAN_LOG), 1, count), TB_NT, 0, s>>>(
-        nullptr, nullptr, nullptr, g0, is_seed, bt);
+        nullptr, nullptr, nullptr, g0, is_seed, bt, nullptr, nullptr);
     return hipGetLastError();
 }
 template <int LGM, int D>
 hipError_t batch_fwd_cols_t(const BlockArgs* bt, uint32_t count, const Geom& g0, const Tables& t, int is_seed,
                             hipStream_t s) {
     k_fwd_cols<LGM, D, true><<<dim3((g0.N2 / (uint32_t)fwd_cols_width(LGM)) * (uint32_t)D, 1, count), FWD_NT, 0, s>>>(
-        nullptr, nullptr, g0, t.tw14, t.lo, t.hi, t.cls, bt, is_seed);
+        nullptr, nullptr, g0, t.tw14, t.lo, t.hi, t.cls, bt, is_seed, nullptr, nullptr);
     return hipGetLastError();
 }
 template <int LG1>

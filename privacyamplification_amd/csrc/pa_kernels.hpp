@@ -52,9 +52,12 @@ __host__ __device__ constexpr uint64_t nib_bytes(uint32_t N, uint32_t W) { retur
 __host__ __device__ constexpr int small_threads(int lgn) { return clamp_threads(2 * (1 << lgn) / 16); }
 
 hipError_t launch_zero_u32(uint32_t* p, uint32_t count, hipStream_t s);
+// bits2/tb2 (and tb2/Y2) non-null: the seed is transformed in the same launch (grid.y = 2),
+// its popcount into count + POP_SLOTS.
 hipError_t launch_transpose(const uint32_t* bits, uint32_t* tb, uint32_t* count, const Geom& g,
-                            int is_seed, hipStream_t s);
-hipError_t launch_fwd_cols(const uint32_t* tb, float2* Y, const Geom& g, const Tables& t, hipStream_t s);
+                            int is_seed, hipStream_t s, const uint32_t* bits2 = nullptr, uint32_t* tb2 = nullptr);
+hipError_t launch_fwd_cols(const uint32_t* tb, float2* Y, const Geom& g, const Tables& t, hipStream_t s,
+                           const uint32_t* tb2 = nullptr, float2* Y2 = nullptr);
 // ROWS_SEEDPREP: S <- split spectrum of the seed rows held in S (in place, N+1 values).
 // ROWS_STATIC:   Yk <- R' from the key rows in Yk and the split seed spectrum S.
 // ROWS_DYN:      Yk <- R' from the key rows in Yk and the seed rows (pass-A output) in S.
