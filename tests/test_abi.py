@@ -9,8 +9,8 @@ import pytest
 ROOT = pathlib.Path(__file__).resolve().parents[1]
 
 
-def declared_functions():
-    text = (ROOT / "include/pa_engine.h").read_text()
+def declared_functions(header="pa_engine.h"):
+    text = (ROOT / "include" / header).read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     names = re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\*?\s+(pa_[a-z0-9_]+)\s*\(", text, re.M)
     return sorted(set(names))
@@ -65,3 +65,36 @@ def test_no_cpu_fallback_when_library_missing(tmp_path, monkeypatch):
     monkeypatch.setattr(E, "_LIB", tmp_path / "libpa_hip.so")
     with pytest.raises(RuntimeError, match="no CPU fallback"):
         E.load_library()
+
+
+def test_zmtp_library_exports_every_declared_symbol():
+    import ctypes
+    lib = ROOT / "privacyamplification_amd" / "libpa_zmtp.so"
+    if not lib.exists():
+        pytest.skip("libpa_zmtp.so not built")
+    L = ctypes.CDLL(str(lib))
+    names = declared_functions("pa_zmtp.h")
+    assert {"pa_zmtp_open", "pa_zmtp_send", "pa_zmtp_recv", "pa_sha3_256"} <= set(names)
+    assert not [n for n in names if not hasattr(L, n)]
+
+
+def _server(tmp_path, cfg_text, *args):
+    import subprocess
+    exe = ROOT / "privacyamplification_amd" / "pa_server"
+    if not exe.exists():
+        pytest.skip("pa_server not built")
+    cfg = tmp_path / "config.yaml"
+    if cfg_text is not None:
+        cfg.write_text(cfg_text)
+    return subprocess.run([str(exe), "--config", str(cfg), *args], cwd=tmp_path, capture_output=True,
+                          text=True, timeout=60)
+
+
+def test_server_config_errors_exit_like_the_reference(tmp_path):
+    """The front-end's exit codes before any device work (PrivacyAmplification.cu:1228-1242, 1415):
+    102 for a missing or malformed config.yaml, 400 for an unusable factor_exp."""
+    assert _server(tmp_path, None).returncode == 102
+    assert _server(tmp_path, "factor_exp 27\n").returncode == 102          # no colon
+    assert _server(tmp_path, "do_compress: maybe\n").returncode == 102     # bad boolean
+    assert _server(tmp_path, "factor_exp: 5  # too small\n").returncode == 400 & 0xFF  # 8-bit status
+    assert _server(tmp_path, "factor_exp: 27\nverify_ampout: true\nverify_ampout_sha3: 'abc'\n").returncode == 102
