@@ -19,6 +19,7 @@
 // (pa_host_alloc); resident seeds (pa_seed_put) serve reused seeds; outputs leave in block order.
 //
 //   pa_server [--config config.yaml] [--factor_exp N] [--blocks K] [speedtest]
+// (--factor_exp overrides the config; --blocks K ends the process after K blocks.)
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -223,6 +224,41 @@ bool parse_hex32(const std::string& h, uint8_t out[32]) {
     return true;
 }
 
+// argv "speedtest" (main, PrivacyAmplification.cu:1631-1643; timing :1758-1777, 2077-2089): with a
+// fresh seed per block (i = 0, reuse 0) and with a resident one (i = 1, reuse -1), for every
+// n = 2^j, j = 10..27, m = 3n/8: 4 untimed blocks, then 10 timed ones, each printed as
+// d[i,j,k]=<Mbit/s of input key> (n over the block's wall time, host buffers in and out).
+int run_speedtest(pa_engine* eng) {
+    constexpr uint32_t JMAX = 27;
+    const uint64_t nmax = 1ull << JMAX;
+    uint32_t *key = nullptr, *seed = nullptr;
+    uint8_t* out = nullptr;
+    int rc = pa_host_alloc(eng, (nmax / 32 + 2) * 4, reinterpret_cast<void**>(&key));
+    if (rc == PA_OK) rc = pa_host_alloc(eng, nmax / 8, reinterpret_cast<void**>(&seed));
+    if (rc == PA_OK) rc = pa_host_alloc(eng, nmax / 16, reinterpret_cast<void**>(&out));
+    for (int i = 0; i < 2 && rc == PA_OK; ++i) {
+        for (uint32_t j = 10; j <= JMAX && rc == PA_OK; ++j) {
+            const uint32_t n = 1u << j, m = n / 4 + n / 8;
+            pa_synth_bits(0x1000 + j, 0.5, uint64_t(n) + 1, key);
+            pa_synth_bits(0x2000 + j, 0.5, n, seed);
+            pa_seed_id sid = 0;
+            if (i == 1 && (rc = pa_seed_put(eng, seed, n, m, -1, &sid)) != PA_OK) break;
+            for (int k = -4; k < 10 && rc == PA_OK; ++k) {
+                const auto t0 = std::chrono::steady_clock::now();
+                rc = i ? pa_run_block_seeded(eng, key, sid, n, m, PA_XOR_KEY_REST | PA_COMPRESS, out)
+                       : pa_run_block(eng, key, seed, n, m, PA_XOR_KEY_REST | PA_COMPRESS, out);
+                const double us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+                if (rc == PA_OK && k >= 0) say("d[%d,%u,%d]=%.3f\n", i, j, k, n / us);
+            }
+            if (sid) pa_seed_drop(eng, sid);
+        }
+    }
+    if (rc != PA_OK) say("speedtest failed: %s\n", pa_strerror(rc));
+    for (void* p : {static_cast<void*>(key), static_cast<void*>(seed), static_cast<void*>(out)})
+        if (p) pa_host_free(eng, p);
+    return rc == PA_OK ? 0 : 1;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -244,12 +280,6 @@ int main(int argc, char** argv) {
         return 102;
     }
     if (factor_override >= 0) cfg.factor_exp = uint32_t(factor_override);
-    if (speedtest) {
-        cfg.factor_exp = 27;
-        cfg.verify_ampout = cfg.use_matrix_seed_server = cfg.use_key_server = cfg.host_ampout_server = false;
-        cfg.store_first_ampouts_in_file = 0;
-        if (max_blocks < 0) max_blocks = 32;
-    }
     if (cfg.factor_exp < 10 || cfg.factor_exp > 29) {
         say("factor_exp %u is outside 10..29 => terminating!\n", cfg.factor_exp);
         return 400;
@@ -274,6 +304,11 @@ int main(int argc, char** argv) {
     if (rc != PA_OK) {
         say("Engine start failed: %s => terminating!\n", pa_strerror(rc));
         return 1;
+    }
+    if (speedtest) {
+        const int code = run_speedtest(eng);
+        pa_destroy(eng);
+        return code;
     }
     pa_serve_config sc{};
     sc.slots = 3;
@@ -316,14 +351,12 @@ int main(int argc, char** argv) {
     if (!cfg.use_key_server) {
         file_key.assign(nw + 2, 0u);
         int code = 0;
-        if (speedtest) pa_synth_bits(0x1000, 0.5, uint64_t(n) + 1, file_key.data());
-        else if (!read_file(cfg.keyfile_path, file_key.data(), (nw + 1) * 4ull, 105, 106, code)) return code;
+        if (!read_file(cfg.keyfile_path, file_key.data(), (nw + 1) * 4ull, 105, 106, code)) return code;
     }
     if (!cfg.use_matrix_seed_server) {
         file_seed.assign(nw, 0u);
         int code = 0;
-        if (speedtest) pa_synth_bits(0x2000, 0.5, n, file_seed.data());
-        else if (!read_file(cfg.toeplitz_seed_path, file_seed.data(), nw * 4ull, 103, 104, code)) return code;
+        if (!read_file(cfg.toeplitz_seed_path, file_seed.data(), nw * 4ull, 103, 104, code)) return code;
     }
     std::thread key_rx([&] {
         while (true) {
@@ -391,8 +424,6 @@ int main(int argc, char** argv) {
         int32_t to_store = cfg.store_first_ampouts_in_file;
         if (to_store != 0) ampout.open("ampout.bin", std::ios::binary);
         auto t_prev = std::chrono::steady_clock::now();
-        uint64_t sent = 0, bits = 0;
-        const auto t0 = t_prev;
         Done d;
         while (done_q.pop(d)) {
             if (d.ticket) {
@@ -425,11 +456,13 @@ int main(int argc, char** argv) {
                 const auto now = std::chrono::steady_clock::now();
                 const double ms = std::chrono::duration<double, std::milli>(now - t_prev).count();
                 t_prev = now;
-                ++sent;
-                bits += d.n;
-                if (cfg.show_ampout >= 0 && !speedtest) {
+                if (cfg.show_ampout >= 0) {
                     say("Blocktime: %.3f ms => %.1f Mbit/s\n", ms, d.n / (ms * 1e3));
-                    for (int i = 0; i < cfg.show_ampout && size_t(i) < bytes; ++i) say("0x%02X\n", o[i]);
+                    for (int i = 0; i < cfg.show_ampout && size_t(i) < bytes; ++i) {  // byte and its bits, MSB first
+                        char bin[9] = {};
+                        for (int k = 0; k < 8; ++k) bin[k] = char('0' + ((o[i] >> (7 - k)) & 1));
+                        say("0x%02X: %s\n", o[i], bin);
+                    }
                 }
                 free_outs.push(d.out_buf);
                 free_keys.push(d.key_buf);
@@ -437,11 +470,6 @@ int main(int argc, char** argv) {
             if (d.seed_buf >= 0) free_seeds.push(d.seed_buf);
             if (d.drop_seed) pa_seed_drop(eng, d.drop_seed);
             if (d.last) break;
-        }
-        if (speedtest && sent) {
-            const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-            say("speedtest: %llu blocks of 2^%u bits in %.3f s => %.2f Gbit/s of input key\n",
-                (unsigned long long)sent, cfg.factor_exp, s, bits / s / 1e9);
         }
         free_keys.close();
         free_seeds.close();

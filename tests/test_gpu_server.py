@@ -84,3 +84,63 @@ verify_ampout: false
     for i, (g, r) in enumerate(zip(got, refs)):
         assert len(g) == 1 and g[0] == r.tobytes(), (i, len(g[0]), r.size)
     assert (tmp_path / "ampout.bin").read_bytes() == refs[0].tobytes() + refs[1].tobytes()
+
+
+def _file_config(tmp_path, e, m, extra):
+    (tmp_path / "config.yaml").write_text(f"""
+factor_exp: {e}
+reuse_seed_amount: -1
+vertical_len: {m}
+do_xor_key_rest: true
+do_compress: true
+show_ampout: 0
+use_matrix_seed_server: false
+toeplitz_seed_path: 'seed.bin'
+use_key_server: false
+keyfile_path: 'key.bin'
+host_ampout_server: false
+store_first_ampouts_in_file: -1
+{extra}
+""")
+
+
+def test_pa_server_file_inputs_and_verify(tmp_path, oracle):
+    """File inputs (keyfile_path, toeplitz_seed_path), ampout.bin for every block
+    (store_first_ampouts_in_file -1), and verify_ampout against a SHA3-256 digest: the right digest
+    passes every block, a wrong one ends the process with the reference's code 101."""
+    import hashlib
+    exe = ROOT / "privacyamplification_amd" / "pa_server"
+    e, m = 18, 3 << 15
+    n = 1 << e
+    key = oracle.synth_bits(0xA1, n + 1)
+    seed = oracle.synth_bits(0xA2, n)
+    (tmp_path / "key.bin").write_bytes(key.tobytes())
+    (tmp_path / "seed.bin").write_bytes(seed.tobytes())
+    ref = oracle.amp_out(key, seed, n, m, method=oracle.NTT).tobytes()
+    digest = hashlib.sha3_256(ref).hexdigest()
+    print("[case] pa_server files + verify n=2^18", flush=True)
+    _file_config(tmp_path, e, m, f"verify_ampout: true\nverify_ampout_sha3: '{digest}'")
+    r = subprocess.run([str(exe), "--blocks", "3"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.count("VERIFIED!") == 3 and r.stdout.count("Blocktime:") == 3
+    assert (tmp_path / "ampout.bin").read_bytes() == ref * 3
+    bad = ("0" if digest[0] != "0" else "1") + digest[1:]
+    _file_config(tmp_path, e, m, f"verify_ampout: true\nverify_ampout_sha3: '{bad}'")
+    r = subprocess.run([str(exe), "--blocks", "3"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 101 and "VERIFICATION FAILED!" in r.stdout, r.stdout + r.stderr
+
+
+def test_pa_server_speedtest(tmp_path):
+    """argv speedtest in the reference's output format: d[i,j,k]=Mbit/s for a fresh seed per block
+    (i=0) and a resident seed (i=1), n = 2^j for j = 10..27, 10 timed blocks each."""
+    import re
+    exe = ROOT / "privacyamplification_amd" / "pa_server"
+    (tmp_path / "config.yaml").write_text("show_zeromq_status: false\n")
+    print("[case] pa_server speedtest", flush=True)
+    r = subprocess.run([str(exe), "speedtest"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    d = {tuple(map(int, k.split(","))): float(v) for k, v in re.findall(r"^d\[([\d,]+)\]=([\d.]+)$", r.stdout, re.M)}
+    assert set(d) == {(i, j, k) for i in (0, 1) for j in range(10, 28) for k in range(10)}
+    assert all(v > 0 for v in d.values())
+    print("speedtest 2^27 Mbit/s: dynamic %.0f, resident %.0f" % (
+        sorted(d[0, 27, k] for k in range(10))[5], sorted(d[1, 27, k] for k in range(10))[5]))
