@@ -311,6 +311,17 @@ __device__ __forceinline__ float2 c2r_split(float2 pk, float2 pp, float2 w) {
 // 2 X[N] (Nyquist) of the length-2N real transform from Z[0], matching r2c_split's scale.
 __device__ __forceinline__ float2 r2c_nyquist(float2 z0) { return make_float2(2.f * (z0.x - z0.y), 0.f); }
 
+
+// Streaming stores (global_store ... nt) for data that the next pass reads only after the whole
+// pass is done: spectra and output nibbles far larger than L2.  Marking them non-temporal keeps
+// them from evicting the lines that concurrent workgroups still share (bit words, twiddles).
+__device__ __forceinline__ void st_nt(float2* p, float2 v) {
+    typedef float f2v __attribute__((ext_vector_type(2)));
+    f2v x = {v.x, v.y};
+    __builtin_nontemporal_store(x, reinterpret_cast<f2v*>(p));
+}
+__device__ __forceinline__ void st_nt_u8(uint8_t* p, uint8_t v) { __builtin_nontemporal_store(v, p); }
+
 // Wave-level max of a non-negative float published to a device float with an atomic max
 // on its bit pattern (non-negative IEEE floats order like their unsigned bit patterns).
 __device__ __forceinline__ void margin_publish(float v, float* margin) {

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <type_traits>
 
 #include "pa_device.hpp"
 #include "pa_geom.hpp"
@@ -178,21 +179,23 @@ __global__ __launch_bounds__(FWD_NT) void k_fwd_cols(const uint32_t* __restrict_
     __shared__ float2 TU[M];                // w_N1^{u c}
     const uint32_t logical = xcd_logical(blockIdx.x, gridDim.x);
     const uint32_t c = logical % D, grp = logical / D;  // columns [CPW grp, CPW grp + CPW)
-    load_twl(twl, tw14, NT);
-    {
-        const float2* src = cls + cls_table_offset(D) + (size_t)c * (NB << BW);
-        for (int e = threadIdx.x; e < NB << BW; e += NT) TB[e >> BW][e & (int)MASK] = src[e];
-    }
-    // w_N1^{u c}: from the 2^14 table while N1 <= 2^14, else from the length-L two-level table
-    if (g.N1 <= (uint32_t)TW14) {
-        const uint32_t step_n1 = TW14 / g.N1;
-        for (int u = threadIdx.x; u < M; u += NT) TU[u] = tw14[((uint32_t)u * c) * step_n1];
-    } else {
-        const uint32_t step_l = g.L / g.N1;
-        for (int u = threadIdx.x; u < M; u += NT) TU[u] = tw_two_level(twlo, twhi, ((uint32_t)u * c) * step_l);
-    }
+    auto fill_tables = [&]() {
+        load_twl(twl, tw14, NT);
+        {
+            const float2* src = cls + cls_table_offset(D) + (size_t)c * (NB << BW);
+            for (int e = threadIdx.x; e < NB << BW; e += NT) TB[e >> BW][e & (int)MASK] = src[e];
+        }
+        // w_N1^{u c}: from the 2^14 table while N1 <= 2^14, else from the length-L two-level table
+        if (g.N1 <= (uint32_t)TW14) {
+            const uint32_t step_n1 = TW14 / g.N1;
+            for (int u = threadIdx.x; u < M; u += NT) TU[u] = tw14[((uint32_t)u * c) * step_n1];
+        } else {
+            const uint32_t step_l = g.L / g.N1;
+            for (int u = threadIdx.x; u < M; u += NT) TU[u] = tw_two_level(twlo, twhi, ((uint32_t)u * c) * step_l);
+        }
+    };
     // the element z(col, u) of this class (both halves of tb: even and odd bits)
-    auto zval = [&](uint32_t t, int u) {
+    auto zval = [&](uint32_t t, int u, auto is_c0) {
         float2 sE = TB[0][t & MASK], sO = TB[0][(t >> 16) & MASK];
 #pragma unroll
         for (int q = 1; q < NB; ++q) {
@@ -200,7 +203,7 @@ __global__ __launch_bounds__(FWD_NT) void k_fwd_cols(const uint32_t* __restrict_
             sO = cadd(sO, TB[q][(t >> (16 + BW * q)) & MASK]);
         }
         float2 z = make_float2(sE.x - sO.y, sE.y + sO.x);
-        if (c == 0) {
+        if constexpr (decltype(is_c0)::value) {
             z.x -= 0.5f * (float)D;
             z.y -= 0.5f * (float)D;
         } else {
@@ -220,15 +223,20 @@ __global__ __launch_bounds__(FWD_NT) void k_fwd_cols(const uint32_t* __restrict_
 #pragma unroll
             for (int r = 0; r < 16; ++r) tv[i][r] = tb[(uint32_t)(j + 32 * r) * g.N2 + t2];
         }
+        fill_tables();  // its loads go out after the bit words: both latencies overlap
         __syncthreads();  // tables
+        auto stage1 = [&](auto is_c0) {
 #pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            const int j = (threadIdx.x >> 4) + 16 * i;
-            float2 v[16];
+            for (int i = 0; i < 2; ++i) {
+                const int j = (threadIdx.x >> 4) + 16 * i;
+                float2 v[16];
 #pragma unroll
-            for (int r = 0; r < 16; ++r) v[r] = zval(tv[i][r], j + 32 * r);
-            stage_first_to_lds<16>(v, buf + col * SEQS, j);
-        }
+                for (int r = 0; r < 16; ++r) v[r] = zval(tv[i][r], j + 32 * r, is_c0);
+                stage_first_to_lds<16>(v, buf + col * SEQS, j);
+            }
+        };
+        if (c == 0) stage1(std::true_type{});
+        else stage1(std::false_type{});
         __syncthreads();
         // stage 2 (radix 32, last): butterfly j = t/16, outputs k1' = j + 16 r
         const int j = threadIdx.x >> 4;
@@ -243,10 +251,11 @@ __global__ __launch_bounds__(FWD_NT) void k_fwd_cols(const uint32_t* __restrict_
             if ((r & 7) == 0 && r)  // re-anchor every 8 steps to bound the recurrence error
                 tw = cmul(base, tw_two_level(twlo, twhi, (uint32_t)((2ull * t2 * 16ull * D * r) & (g.L - 1u))));
             const uint32_t k1 = c + (uint32_t)D * (uint32_t)(j + 16 * r);
-            Y[(size_t)k1 * g.N2 + t2] = cmul(F[r], tw);
+            st_nt(&Y[(size_t)k1 * g.N2 + t2], cmul(F[r], tw));
             tw = cmul(tw, step);
         }
     } else {
+        fill_tables();
         constexpr int IPT = (CPW * M + NT - 1) / NT;
         uint32_t tbv[IPT];
 #pragma unroll
@@ -258,7 +267,9 @@ __global__ __launch_bounds__(FWD_NT) void k_fwd_cols(const uint32_t* __restrict_
 #pragma unroll
         for (int i = 0; i < IPT; ++i) {
             const int e = threadIdx.x + i * NT;
-            if (e < CPW * M) buf[(e % CPW) * SEQS + padi(e / CPW)] = zval(tbv[i], e / CPW);
+            if (e < CPW * M)
+                buf[(e % CPW) * SEQS + padi(e / CPW)] =
+                    c == 0 ? zval(tbv[i], e / CPW, std::true_type{}) : zval(tbv[i], e / CPW, std::false_type{});
         }
         __syncthreads();
         lds_fft<LGM, CPW, SEQS, NT>(buf, twl);
@@ -266,7 +277,7 @@ __global__ __launch_bounds__(FWD_NT) void k_fwd_cols(const uint32_t* __restrict_
             const int col = e % CPW, k1p = e / CPW;
             const uint32_t k1 = c + (uint32_t)D * (uint32_t)k1p, t2 = grp * CPW + (uint32_t)col;
             const float2 tw = tw_two_level(twlo, twhi, (uint32_t)((2ull * t2 * k1) & (g.L - 1u)));
-            Y[(size_t)k1 * g.N2 + t2] = cmul(buf[col * SEQS + padi(k1p)], tw);
+            st_nt(&Y[(size_t)k1 * g.N2 + t2], cmul(buf[col * SEQS + padi(k1p)], tw));
         }
     }
 }
@@ -414,8 +425,8 @@ __global__ __launch_bounds__(ROWS_NT) void k_rows(float2* __restrict__ Yk, float
                 }
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
-                    Sst[(size_t)m.rowA * N2 + m.jA + 512 * r] = SA[r];
-                    Sst[(size_t)m.rowB * N2 + m.jB + 512 * r] = SB[r];
+                    st_nt(&Sst[(size_t)m.rowA * N2 + m.jA + 512 * r], SA[r]);
+                    st_nt(&Sst[(size_t)m.rowB * N2 + m.jB + 512 * r], SB[r]);
                 }
                 if (m.special) Sst[(size_t)N1 * N2] = Sn;
                 if (!more) break;
@@ -496,7 +507,7 @@ __global__ __launch_bounds__(ROWS_NT) void k_rows(float2* __restrict__ Yk, float
             for (int r = 0; r < 32; ++r) {
                 if ((r & 7) == 0 && r)  // re-anchor every 8 steps to bound the recurrence error
                     tw = cmul(base, anchor[r / 8 - 1]);
-                op[256 * r] = cmul(F[r], tw);
+                st_nt(&op[256 * r], cmul(F[r], tw));
                 tw = cmul(tw, step);
             }
         }
@@ -581,7 +592,7 @@ __global__ __launch_bounds__(inv_cols_threads(LG1)) void k_inv_cols(
                 const float2 G = buf[padi(k)];
                 const uint32_t b0 = round_bit(G.x * invN + corr, dist);
                 const uint32_t b1 = round_bit(-G.y * invN + corr, dist);
-                nibT[(size_t)t2 * N1 + t1] = (uint8_t)((b0 << 1) | b1);
+                st_nt_u8(&nibT[(size_t)t2 * N1 + t1], (uint8_t)((b0 << 1) | b1));
             }
             __syncthreads();  // reads done before the next half is written
         }
@@ -630,7 +641,7 @@ __global__ __launch_bounds__(inv_cols_threads(LG1)) void k_inv_cols(
                 const uint32_t b1 = round_bit(-G0[r].y * invN + corr, dist);
                 const uint32_t b2 = round_bit(G1[r].x * invN + corr, dist);
                 const uint32_t b3 = round_bit(-G1[r].y * invN + corr, dist);
-                nibT[(size_t)(t2 >> 1) * N1 + t1] = (uint8_t)((b0 << 3) | (b1 << 2) | (b2 << 1) | b3);
+                st_nt_u8(&nibT[(size_t)(t2 >> 1) * N1 + t1], (uint8_t)((b0 << 3) | (b1 << 2) | (b2 << 1) | b3));
             }
         }
         if (margin) margin_publish(dist, margin);
