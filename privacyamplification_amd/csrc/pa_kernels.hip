@@ -8,7 +8,8 @@
 //
 //   k_transpose_bits  key/seed RefBits -> per-(class,column) bit masks for pass A, plus popcounts
 //                     (binInt2float's popcount, PrivacyAmplification.cu:570-602)
-//   k_fwd_cols        pass A: decimated column FFTs of the +-1/2-encoded bits (R2C forward part)
+//   k_fwd_cols        pass A: decimated column FFTs of the +-1/2-encoded bits (R2C forward part);
+//   k_fwd_cols9p      its persistent form for 512-point classes (the 2^27-bit block's shape)
 //   k_rows            pass B: row FFTs, real split, DC zeroing (setFirstElementToZero, :408-418),
 //                     complex product (ElementWiseProduct, :465-481), inverse split, inverse row FFTs
 //   k_inv_cols        pass A': inverse column FFTs + round-mod-2 with the correction term
@@ -280,6 +281,187 @@ __global__ __launch_bounds__(FWD_NT) void k_fwd_cols(const uint32_t* __restrict_
             st_nt(&Y[(size_t)k1 * g.N2 + t2], cmul(buf[col * SEQS + padi(k1p)], tw));
         }
     }
+}
+
+
+// Raw buffer access (vector memory instructions): one per-lane byte offset plus a uniform
+// scalar offset, so a run of accesses at fixed strides keeps a single address register.
+typedef unsigned int pa_u2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ void buf_store_nt(float2 v, __amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff) {
+    const pa_u2v x = {__float_as_uint(v.x), __float_as_uint(v.y)};
+    __builtin_amdgcn_raw_buffer_store_b64(x, rs, (int)voff, (int)soff, 2);  // aux 2: nt
+}
+__device__ __forceinline__ uint32_t buf_load_u32(__amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff) {
+    return __builtin_amdgcn_raw_buffer_load_b32(rs, (int)voff, (int)soff, 0);
+}
+
+// Pieces of the persistent pass A (k_fwd_cols9p).  Thread (col = t mod 16, jt = t / 16).
+// Bit words of column group grp: stage-1 butterflies j = jt + 16 i, inputs u = j + 32 r.
+__device__ __forceinline__ void fwd9_load_tv(uint32_t (&tv)[2][16], __amdgpu_buffer_rsrc_t rtb, uint32_t N2,
+                                             uint32_t t2, int jt) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const uint32_t voff = ((uint32_t)(jt + 16 * i) * N2 + t2) * 4u;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) tv[i][r] = buf_load_u32(rtb, voff, (uint32_t)(32 * r) * N2 * 4u);
+    }
+}
+// Table entries of the output twiddles w_N^{t2 k1}, k1 = c + D (jt + 16 r) = base step^r, with
+// anchors w_N^{t2 16 D 8 (a+1)}: (lo, hi) pairs of base, step, anchor 0..2.  The products are
+// formed where they are used, so that gathering them never waits on other loads.
+template <int D>
+__device__ __forceinline__ void fwd9_load_tw(float2 (&tw)[10], const float2* __restrict__ lo,
+                                             const float2* __restrict__ hi, uint32_t L, uint32_t c, uint32_t t2,
+                                             int jt) {
+    const uint64_t T = t2;
+    uint32_t x[5];
+    x[0] = (uint32_t)((2ull * T * (c + (uint64_t)D * (uint32_t)jt)) & (L - 1u));
+    x[1] = (uint32_t)((2ull * T * 16ull * D) & (L - 1u));
+#pragma unroll
+    for (int a = 0; a < 3; ++a) x[2 + a] = (uint32_t)((2ull * T * 16ull * D * (8 * (a + 1))) & (L - 1u));
+#pragma unroll
+    for (int e = 0; e < 5; ++e) {
+        tw[2 * e] = lo[x[e] & (TW14 - 1)];
+        tw[2 * e + 1] = hi[x[e] >> TW14_LOG];
+    }
+}
+template <int D, bool C0>
+__device__ __forceinline__ void fwd9_stage1(const uint32_t (&tv)[2][16], float2* buf,
+                                            const float2 (*TB)[1 << cls_bits(D)], const float2* TU, int col, int jt) {
+    constexpr int SEQS = seq_stride(512);
+    constexpr int BW = cls_bits(D);
+    constexpr int NB = cls_tables(D);
+    constexpr uint32_t MASK = (1u << BW) - 1u;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int j = jt + 16 * i;
+        float2 v[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const uint32_t t = tv[i][r];
+            float2 sE = TB[0][t & MASK], sO = TB[0][(t >> 16) & MASK];
+#pragma unroll
+            for (int q = 1; q < NB; ++q) {
+                sE = cadd(sE, TB[q][(t >> (BW * q)) & MASK]);
+                sO = cadd(sO, TB[q][(t >> (16 + BW * q)) & MASK]);
+            }
+            float2 z = make_float2(sE.x - sO.y, sE.y + sO.x);
+            if constexpr (C0) {
+                z.x -= 0.5f * (float)D;
+                z.y -= 0.5f * (float)D;
+            } else {
+                z = cmul(z, TU[j + 32 * r]);
+            }
+            v[r] = z;
+        }
+        stage_first_to_lds<16>(v, buf + col * SEQS, j);
+    }
+}
+template <int D>
+__device__ __forceinline__ void fwd9_stage2(const float2* buf, const float2* twl, const float2 (&tw)[10],
+                                            __amdgpu_buffer_rsrc_t ry, uint32_t N2, uint32_t c, uint32_t t2,
+                                            int col, int jt) {
+    constexpr int SEQS = seq_stride(512);
+    float2 F[32];
+    stage_last_from_lds<32, 512>(F, buf + col * SEQS, jt, twl);
+    const float2 base = cmul(tw[0], tw[1]), step = cmul(tw[2], tw[3]);
+    const uint32_t voff = ((c + (uint32_t)D * (uint32_t)jt) * N2 + t2) * 8u;
+    float2 w = base;
+#pragma unroll
+    for (int r = 0; r < 32; ++r) {
+        if ((r & 7) == 0 && r) w = cmul(base, cmul(tw[2 * (1 + r / 8)], tw[2 * (1 + r / 8) + 1]));
+        buf_store_nt(cmul(F[r], w), ry, voff, (uint32_t)(16 * D * r) * N2 * 8u);
+        w = cmul(w, step);
+    }
+}
+template <int D, bool C0>
+__device__ __forceinline__ void fwd9_walk(__amdgpu_buffer_rsrc_t rtb, __amdgpu_buffer_rsrc_t ry, const Geom& g,
+                                          const float2* __restrict__ twlo, const float2* __restrict__ twhi,
+                                          float2* buf, const float2* twl, const float2 (*TB)[1 << cls_bits(D)],
+                                          const float2* TU, uint32_t c, uint32_t grp0, uint32_t per) {
+    const int col = threadIdx.x & 15;
+    const int jt = threadIdx.x >> 4;
+    const uint32_t N2 = g.N2;
+    uint32_t tv[2][16];
+    float2 tw[10], twn[10];
+    // group grp0, peeled: every entry into the loop below then has the same memory-operation
+    // history (next group's loads, then this group's stores), so the compiler's waits on the
+    // prefetched registers count past the stores instead of draining them
+    fwd9_load_tv(tv, rtb, N2, grp0 * 16u + (uint32_t)col, jt);
+    fwd9_load_tw<D>(tw, twlo, twhi, g.L, c, grp0 * 16u + (uint32_t)col, jt);
+    __syncthreads();  // tables
+    fwd9_stage1<D, C0>(tv, buf, TB, TU, col, jt);
+    __syncthreads();
+    if (per > 1) {
+        fwd9_load_tv(tv, rtb, N2, (grp0 + 1u) * 16u + (uint32_t)col, jt);
+        fwd9_load_tw<D>(twn, twlo, twhi, g.L, c, (grp0 + 1u) * 16u + (uint32_t)col, jt);
+    }
+    fwd9_stage2<D>(buf, twl, tw, ry, N2, c, grp0 * 16u + (uint32_t)col, col, jt);
+    for (uint32_t it = 1; it < per; ++it) {
+        const uint32_t grp = grp0 + it;
+#pragma unroll
+        for (int e = 0; e < 10; ++e) tw[e] = twn[e];
+        __syncthreads();  // stage-2 reads of the previous group are done
+        fwd9_stage1<D, C0>(tv, buf, TB, TU, col, jt);
+        __syncthreads();
+        if (it + 1 < per) {
+            fwd9_load_tv(tv, rtb, N2, (grp + 1u) * 16u + (uint32_t)col, jt);
+            fwd9_load_tw<D>(twn, twlo, twhi, g.L, c, (grp + 1u) * 16u + (uint32_t)col, jt);
+        }
+        fwd9_stage2<D>(buf, twl, tw, ry, N2, c, grp * 16u + (uint32_t)col, col, jt);
+    }
+}
+
+// Persistent pass A for M = 512 (the fused 16 x 32 plan of k_fwd_cols): workgroup L of G keeps one
+// class c = L mod D and walks the column groups [k per, k per + per), k = L / D, so its class
+// tables are loaded once, and each group's stores drain while the next group is transformed (its
+// loads were issued before those stores).  The D workgroups of one k sit on one XCD and walk the
+// same groups in step, so each bit word is fetched into L2 once.
+template <int D>
+__global__ __launch_bounds__(FWD_NT, 2) void k_fwd_cols9p(const uint32_t* __restrict__ tb,
+                                                          float2* __restrict__ Y, Geom g,
+                                                          const float2* __restrict__ tw14,
+                                                          const float2* __restrict__ twlo,
+                                                          const float2* __restrict__ twhi,
+                                                          const float2* __restrict__ cls,
+                                                          const uint32_t* __restrict__ tb2,
+                                                          float2* __restrict__ Y2, uint32_t per) {
+    if (blockIdx.y == 1) {
+        tb = tb2;
+        Y = Y2;
+    }
+    constexpr int M = 512;
+    constexpr int NT = FWD_NT;
+    constexpr int SEQS = seq_stride(M);
+    static_assert(NT == 256, "fused pass A expects 256 threads");
+    __shared__ float2 buf[16 * SEQS];
+    __shared__ float2 twl[TWL_SIZE];
+    constexpr int BW = cls_bits(D);
+    constexpr int NB = cls_tables(D);
+    constexpr uint32_t MASK = (1u << BW) - 1u;
+    __shared__ float2 TB[NB][1 << BW];
+    __shared__ float2 TU[M];
+    const uint32_t logical = xcd_logical(blockIdx.x, gridDim.x);
+    const uint32_t c = logical % D, grp0 = (logical / D) * per;
+    const __amdgpu_buffer_rsrc_t rtb = make_rsrc(tb, (uint32_t)M * g.N2 * 4u);
+    const __amdgpu_buffer_rsrc_t ry = make_rsrc(Y, g.N1 * g.N2 * 8u);
+    load_twl(twl, tw14, NT);
+    {
+        const float2* src = cls + cls_table_offset(D) + (size_t)c * (NB << BW);
+        for (int e = threadIdx.x; e < NB << BW; e += NT) TB[e >> BW][e & (int)MASK] = src[e];
+    }
+    if (g.N1 <= (uint32_t)TW14) {
+        const uint32_t step_n1 = TW14 / g.N1;
+        for (int u = threadIdx.x; u < M; u += NT) TU[u] = tw14[((uint32_t)u * c) * step_n1];
+    } else {
+        const uint32_t step_l = g.L / g.N1;
+        for (int u = threadIdx.x; u < M; u += NT) TU[u] = tw_two_level(twlo, twhi, ((uint32_t)u * c) * step_l);
+    }
+    if (c == 0) fwd9_walk<D, true>(rtb, ry, g, twlo, twhi, buf, twl, TB, TU, c, grp0, per);
+    else fwd9_walk<D, false>(rtb, ry, g, twlo, twhi, buf, twl, TB, TU, c, grp0, per);
 }
 
 // ------------------------------------------------------------------ pass B ----
@@ -959,6 +1141,15 @@ namespace {
 template <int LGM, int D>
 hipError_t launch_fwd_cols_t(const uint32_t* tb, float2* Y, const Geom& g, const Tables& t, hipStream_t s,
                              const uint32_t* tb2, float2* Y2) {
+    if constexpr (LGM == 9) {
+        // persistent: 512 workgroups in all (two per CU), each holding one class of one operand
+        const uint32_t groups = g.N2 / 16u, G = Y2 ? 256u : 512u;
+        if ((G % (uint32_t)D) == 0u && ((groups * (uint32_t)D) % G) == 0u) {
+            const uint32_t per = groups * (uint32_t)D / G;
+            k_fwd_cols9p<D><<<dim3(G, Y2 ? 2 : 1), FWD_NT, 0, s>>>(tb, Y, g, t.tw14, t.lo, t.hi, t.cls, tb2, Y2, per);
+            return hipGetLastError();
+        }
+    }
     const uint32_t grid = (g.N2 / (uint32_t)fwd_cols_width(LGM)) * (uint32_t)D;
     k_fwd_cols<LGM, D, false><<<dim3(grid, Y2 ? 2 : 1), FWD_NT, 0, s>>>(tb, Y, g, t.tw14, t.lo, t.hi, t.cls, nullptr, 0,
                                                                     tb2, Y2);
