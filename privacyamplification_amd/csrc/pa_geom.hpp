@@ -30,7 +30,8 @@ struct Geom {
     uint32_t N1, N2, lg1, lg2;         // four-step split (N1 = 1 on the small path)
     uint32_t M, D, W;                  // pass A class length / classes, pass A' columns per workgroup
     uint32_t out_bits, out_off;        // output window in conv indices
-    uint32_t seed_words, seed_rot;     // n/32, rotation of the seed in words (m/32 when L > n)
+    uint32_t seed_words, seed_rot;     // n/32, rotation of the seed in words (m/32 when L > n); the
+                                       // rotated seed repeats with period n up to L
     uint32_t key_hw;                   // h/32: index of the key word masked to its MSB
     uint32_t do_xor, reverse_endian;
     float inv_N;                       // 1/(8N): output normalisation (r2c_split / c2r_split scale)

@@ -36,10 +36,16 @@ __device__ __forceinline__ uint32_t key_word(const uint32_t* __restrict__ key, c
     if (widx == g.key_hw) return key[widx] & 0x80000000u;
     return 0u;
 }
-// Word widx of the seed sequence t: the n seed bits rotated left by seed_rot words, zero beyond n.
+// Word widx (< L/32) of the seed sequence t: the n seed bits rotated left by seed_rot words and
+// repeated with period n up to the transform length L.  The output window only reads t[0, n),
+// where the repetition changes nothing; beyond n it keeps the +-1/2-encoded sequence free of a
+// constant block.  A zero pad there would become a -1/2 block whose convolution with the key's
+// pad swings by about h/4 across positions after the mean is removed, far beyond FP32's
+// resolution at L = 2^24; the repeated seed leaves every position a random-walk sum of the same
+// statistics, and every sum in the correction stays constant (ws counts all L positions).
 __device__ __forceinline__ uint32_t seed_word(const uint32_t* __restrict__ seed, const Geom& g, uint32_t widx) {
-    if (widx >= g.seed_words) return 0u;
-    uint32_t s = widx + g.seed_rot;
+    uint32_t s = widx + g.seed_rot;  // < 3 seed_words: L < 2n when L > n
+    if (s >= g.seed_words) s -= g.seed_words;
     if (s >= g.seed_words) s -= g.seed_words;
     return seed[s];
 }

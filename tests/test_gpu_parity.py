@@ -96,6 +96,32 @@ def test_non_power_of_two_blocks(engine, oracle, n, m):
     assert np.array_equal(engine.amplify(key, seed, n, m), ref)
 
 
+@pytest.mark.parametrize("n,m", [((1 << 23) + 32, 1 << 20), (8664960, 1083104), (10852704, 2713152),
+                                 (20229632, 5057408), (26088768, 6522176), ((1 << 25) - 32, 3 << 22)])
+def test_large_non_power_of_two_blocks(engine, oracle, n, m):
+    """Non-2^k blocks whose transform length L = 2^ceil(log2 n) leaves up to half of L beyond n:
+    the engine repeats the rotated seed up to L, so the values before rounding stay small (a zero
+    pad there makes them swing by about h/4 and FP32 rounding fails from L = 2^24 on).  Checked
+    against the exact NTT oracle, one block at a time and in one batch, with the rounding margin."""
+    import torch
+    say(f"large non-2^k n={n} m={m}")
+    key = oracle.synth_bits(0xA00 + n % 997, n + 1)
+    seed = oracle.synth_bits(0xB00 + n % 991, n)
+    ref = oracle.amp_out(key, seed, n, m, method=oracle.NTT)
+    dev = torch.device("cuda:0")
+    kd = torch.from_numpy(np.concatenate([key, np.zeros(1, np.uint32)]).view(np.int32)).to(dev)
+    sd = torch.from_numpy(seed.view(np.int32)).to(dev)
+    out = torch.zeros(m // 8 + 16, dtype=torch.uint8, device=dev)
+    margin = torch.zeros(1, dtype=torch.float32, device=dev)
+    engine.block_device(kd.data_ptr(), sd.data_ptr(), n, m, 3, out.data_ptr(), 0, margin.data_ptr())
+    torch.cuda.synchronize()
+    assert np.array_equal(out[: m // 8].cpu().numpy(), ref)
+    assert float(margin.item()) < 0.25, float(margin.item())
+    outs = engine.run_batch([{"n": n, "m": m, "key": key, "seed": seed},
+                             {"n": n, "m": m, "key": key, "seed": seed}])
+    assert all(np.array_equal(o, ref) for o in outs)
+
+
 def test_async_submit_wait(engine, oracle):
     n, m = 1 << 16, 3 << 13
     say("async submit x4")
