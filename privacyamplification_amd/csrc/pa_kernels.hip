@@ -426,7 +426,7 @@ __device__ __forceinline__ void fwd9_walk(__amdgpu_buffer_rsrc_t rtb, __amdgpu_b
 // tables are loaded once, and each group's stores drain while the next group is transformed (its
 // loads were issued before those stores).  The D workgroups of one k sit on one XCD and walk the
 // same groups in step, so each bit word is fetched into L2 once.
-template <int D>
+template <int D, bool BATCH>
 __global__ __launch_bounds__(FWD_NT, 2) void k_fwd_cols9p(const uint32_t* __restrict__ tb,
                                                           float2* __restrict__ Y, Geom g,
                                                           const float2* __restrict__ tw14,
@@ -434,8 +434,15 @@ __global__ __launch_bounds__(FWD_NT, 2) void k_fwd_cols9p(const uint32_t* __rest
                                                           const float2* __restrict__ twhi,
                                                           const float2* __restrict__ cls,
                                                           const uint32_t* __restrict__ tb2,
-                                                          float2* __restrict__ Y2, uint32_t per) {
-    if (blockIdx.y == 1) {
+                                                          float2* __restrict__ Y2, uint32_t per,
+                                                          const BlockArgs* __restrict__ bt, int is_seed) {
+    if constexpr (BATCH) {  // grid.z = block of a batch group (one transform length)
+        const BlockArgs& a = bt[blockIdx.z];
+        if (is_seed && a.seed == nullptr) return;  // resident seed: nothing to transform
+        g = a.g;
+        tb = is_seed ? a.tbs : a.tbk;
+        Y = is_seed ? a.Ys : a.Yk;
+    } else if (blockIdx.y == 1) {  // key and seed in one launch: grid.y selects the seed
         tb = tb2;
         Y = Y2;
     }
@@ -1152,7 +1159,8 @@ hipError_t launch_fwd_cols_t(const uint32_t* tb, float2* Y, const Geom& g, const
         const uint32_t groups = g.N2 / 16u, G = Y2 ? 256u : 512u;
         if ((G % (uint32_t)D) == 0u && ((groups * (uint32_t)D) % G) == 0u) {
             const uint32_t per = groups * (uint32_t)D / G;
-            k_fwd_cols9p<D><<<dim3(G, Y2 ? 2 : 1), FWD_NT, 0, s>>>(tb, Y, g, t.tw14, t.lo, t.hi, t.cls, tb2, Y2, per);
+            k_fwd_cols9p<D, false><<<dim3(G, Y2 ? 2 : 1), FWD_NT, 0, s>>>(tb, Y, g, t.tw14, t.lo, t.hi, t.cls, tb2, Y2,
+                                                                         per, nullptr, 0);
             return hipGetLastError();
         }
     }
@@ -1314,6 +1322,15 @@ hipError_t batch_transpose_t(const BlockArgs* bt, uint32_t count, const Geom& g0
 template <int LGM, int D>
 hipError_t batch_fwd_cols_t(const BlockArgs* bt, uint32_t count, const Geom& g0, const Tables& t, int is_seed,
                             hipStream_t s) {
+    if constexpr (LGM == 9) {
+        // persistent form, four column groups per workgroup, one class each
+        const uint32_t groups = g0.N2 / 16u, per = 4u, G = groups * (uint32_t)D / per;
+        if ((G % (uint32_t)D) == 0u && (G % 8u) == 0u) {
+            k_fwd_cols9p<D, true><<<dim3(G, 1, count), FWD_NT, 0, s>>>(nullptr, nullptr, g0, t.tw14, t.lo, t.hi, t.cls,
+                                                                      nullptr, nullptr, per, bt, is_seed);
+            return hipGetLastError();
+        }
+    }
     k_fwd_cols<LGM, D, true><<<dim3((g0.N2 / (uint32_t)fwd_cols_width(LGM)) * (uint32_t)D, 1, count), FWD_NT, 0, s>>>(
         nullptr, nullptr, g0, t.tw14, t.lo, t.hi, t.cls, bt, is_seed, nullptr, nullptr);
     return hipGetLastError();
