@@ -308,6 +308,24 @@ __device__ __forceinline__ float2 c2r_split(float2 pk, float2 pp, float2 w) {
     const float2 t = cmul(conjf2(w), d);
     return make_float2(e.x - t.y, e.y + t.x);  // e + i*t
 }
+// Both splits of a partner pair k, N-k at once.  The partner's twiddle is
+// exp(-2 pi i (N-k)/(2N)) = -conj(w), so its sum, difference and twiddled difference are the
+// conjugate, the negated conjugate and the conjugate of the first one's: one complex product
+// serves both.  r2c_split_pair returns 2 X[k], 2 X[N-k]; c2r_split_pair returns 2 Q[k], 2 Q[N-k].
+__device__ __forceinline__ void r2c_split_pair(float2 zk, float2 zp, float2 w, float2& xk, float2& xp) {
+    const float2 zc = conjf2(zp);
+    const float2 e = cadd(zk, zc);
+    const float2 t = cmul(w, csub(zk, zc));
+    xk = make_float2(e.x + t.y, e.y - t.x);
+    xp = make_float2(e.x - t.y, -(e.y + t.x));
+}
+__device__ __forceinline__ void c2r_split_pair(float2 pk, float2 pp, float2 w, float2& qk, float2& qp) {
+    const float2 pc = conjf2(pp);
+    const float2 e = cadd(pk, pc);
+    const float2 t = cmul(conjf2(w), csub(pk, pc));
+    qk = make_float2(e.x - t.y, e.y + t.x);
+    qp = make_float2(e.x + t.y, t.x - e.y);
+}
 // 2 X[N] (Nyquist) of the length-2N real transform from Z[0], matching r2c_split's scale.
 __device__ __forceinline__ float2 r2c_nyquist(float2 z0) { return make_float2(2.f * (z0.x - z0.y), 0.f); }
 

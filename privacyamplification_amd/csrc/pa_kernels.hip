@@ -594,10 +594,7 @@ __global__ __launch_bounds__(ROWS_NT) void k_rows(float2* __restrict__ Yk, float
             });
             if (!m.special) {
 #pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    SA[r] = r2c_split(zA[r], zB[15 - r], wkA(r));
-                    SB[15 - r] = r2c_split(zB[15 - r], zA[r], wkB(15 - r));
-                }
+                for (int r = 0; r < 16; ++r) r2c_split_pair(zA[r], zB[15 - r], wkA(r), SA[r], SB[15 - r]);
             } else {
                 Sn = r2c_nyquist(zA[0]);
                 SA[0] = make_float2(0.f, 0.f);
@@ -656,9 +653,16 @@ __global__ __launch_bounds__(ROWS_NT) void k_rows(float2* __restrict__ Yk, float
                 qb = conjf2(c2r_split(pb, pa, wb));
             };
             if (!m.special) {
+                // partner pairs (A r, B 15-r): k and N-k, one twiddle for both
 #pragma unroll
-                for (int r = 0; r < 16; ++r)
-                    unit(zA[r], zB[15 - r], SA[r], SB[15 - r], wkA(r), wkB(15 - r), qA[r], qB[15 - r]);
+                for (int r = 0; r < 16; ++r) {
+                    const float2 w = wkA(r);
+                    float2 xa, xb, ua, ub;
+                    r2c_split_pair(zA[r], zB[15 - r], w, xa, xb);
+                    c2r_split_pair(cmul(xa, SA[r]), cmul(xb, SB[15 - r]), w, ua, ub);
+                    qA[r] = conjf2(ua);
+                    qB[15 - r] = conjf2(ub);
+                }
             } else {
                 const float2 pn = cmul(r2c_nyquist(zA[0]), Sn);
                 qA[0] = conjf2(c2r_split(make_float2(0.f, 0.f), pn, make_float2(1.f, 0.f)));
