@@ -59,6 +59,23 @@ def test_random_blocks_vs_oracle(engine, oracle, e, frac, do_xor, do_compress):
     assert np.array_equal(got, ref)
 
 
+@pytest.mark.parametrize("e,frac,do_xor,do_compress", [(24, (3, 8), False, False), (27, (3, 8), True, False),
+                                                       (27, (1, 8), True, True), (26, (1, 2), False, True),
+                                                       (25, (1, 4), True, False)])
+def test_large_blocks_flags_and_ratios(engine, oracle, e, frac, do_xor, do_compress):
+    """The output windows of the other flags and m/n ratios at full transform sizes (pass A'
+    covers n/2 bits without compression, m bits with it), against the NTT oracle."""
+    n = 1 << e
+    m = n * frac[0] // frac[1]
+    say(f"large n=2^{e} m={m} xor={do_xor} compress={do_compress}")
+    key = oracle.synth_bits(0x300 + e, n + 1)
+    seed = oracle.synth_bits(0x400 + e, n)
+    if not do_xor:
+        key = key[: (n - m) // 32 + 1].copy()
+    ref = oracle.amp_out(key, seed, n, m, do_xor, do_compress, method=oracle.NTT)
+    assert np.array_equal(engine.amplify(key, seed, n, m, do_xor, do_compress), ref)
+
+
 @pytest.mark.parametrize("e", [12, 14, 16, 20, 22])
 def test_static_seed_matches_dynamic(engine, oracle, e):
     n, m = 1 << e, 3 * (1 << e) // 8
@@ -71,6 +88,22 @@ def test_static_seed_matches_dynamic(engine, oracle, e):
             ref = oracle.amp_out(key, seed, n, m, method=oracle.NTT)
             assert np.array_equal(engine.amplify_seeded(key, sid, n, m), ref)
             assert np.array_equal(engine.amplify(key, seed, n, m), ref)
+    finally:
+        engine.seed_drop(sid)
+
+
+@pytest.mark.parametrize("n,m", [(3 << 21, 3 << 18), (20229632, 5057408)])
+def test_static_seed_non_power_of_two(engine, oracle, n, m):
+    """A resident seed for a non-2^k block: its spectrum is built from the rotated seed repeated up
+    to L, like the dynamic path's, and serves several keys."""
+    say(f"static seed non-2^k n={n} m={m}")
+    seed = oracle.synth_bits(0x5EEE, n)
+    sid = engine.seed_put(seed, n, m)
+    try:
+        for i in range(2):
+            key = oracle.synth_bits(0x1100 + i, n + 1)
+            ref = oracle.amp_out(key, seed, n, m, method=oracle.NTT)
+            assert np.array_equal(engine.amplify_seeded(key, sid, n, m), ref)
     finally:
         engine.seed_drop(sid)
 
