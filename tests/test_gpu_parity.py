@@ -205,6 +205,32 @@ def test_golden_manifest(engine, big_engine, oracle, item):
     assert sha3(eng.amplify(key, seed, n, m)) == item["sha3"]
 
 
+def test_golden_manifest_batched(engine, oracle):
+    """The batched kernels at the reference's largest size: the four 2^27 manifest blocks (two with
+    their own seeds, two on one resident seed) in one batch, against the committed digests."""
+    items = [it for it in _manifest() if it["n"] == 1 << 27]
+    assert len(items) == 4
+    say("manifest 2^27 blocks in one batch")
+    blocks, sids = [], []
+    try:
+        for it in items:
+            n, m = it["n"], it["m"]
+            key = oracle.synth_bits(it["key_seed"], n + 1, it["p_key"])
+            seed = oracle.synth_bits(it["toeplitz_seed"], n)
+            if "static" in it["name"]:
+                if not sids:  # both static blocks use the same seed: one resident spectrum
+                    sids.append(engine.seed_put(seed, n, m))
+                blocks.append({"n": n, "m": m, "key": key, "seed": None, "sid": sids[0]})
+            else:
+                blocks.append({"n": n, "m": m, "key": key, "seed": seed})
+        outs = engine.run_batch(blocks)
+        for it, o in zip(items, outs):
+            assert sha3(o) == it["sha3"], it["name"]
+    finally:
+        for sid in sids:
+            engine.seed_drop(sid)
+
+
 def test_linearity_at_full_size(engine, oracle):
     """amp_out is GF(2)-linear in the key (T x and the key rest both are): out(a^b) = out(a)^out(b)."""
     n, m = 1 << 27, 3 << 24
