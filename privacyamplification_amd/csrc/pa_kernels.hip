@@ -1159,10 +1159,15 @@ template <int LGM, int D>
 hipError_t launch_fwd_cols_t(const uint32_t* tb, float2* Y, const Geom& g, const Tables& t, hipStream_t s,
                              const uint32_t* tb2, float2* Y2) {
     if constexpr (LGM == 9) {
-        // persistent: 512 workgroups in all (two per CU), each holding one class of one operand
-        const uint32_t groups = g.N2 / 16u, G = Y2 ? 256u : 512u;
-        if ((G % (uint32_t)D) == 0u && ((groups * (uint32_t)D) % G) == 0u) {
-            const uint32_t per = groups * (uint32_t)D / G;
+        // Each workgroup holds one class of one operand and walks `per` column groups: 16 (8 for a
+        // single operand), fewer while that would leave under 256 (512) workgroups per operand.
+        // 1024 workgroups at 2^27, two waves of two per CU: as fast as one persistent wave, and a
+        // CU that cannot take its second workgroup at once delays a short walk, not a long one.
+        const uint32_t groups = g.N2 / 16u, target = Y2 ? 256u : 512u;
+        uint32_t per = Y2 ? 16u : 8u;
+        while (per > 1u && groups * (uint32_t)D / per < target) per >>= 1;
+        const uint32_t G = groups * (uint32_t)D / per;
+        if ((groups % per) == 0u) {
             k_fwd_cols9p<D, false><<<dim3(G, Y2 ? 2 : 1), FWD_NT, 0, s>>>(tb, Y, g, t.tw14, t.lo, t.hi, t.cls, tb2, Y2,
                                                                          per, nullptr, 0);
             return hipGetLastError();
