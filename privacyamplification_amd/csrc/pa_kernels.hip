@@ -147,11 +147,11 @@ __global__ __launch_bounds__(TB_NT) void k_transpose_bits(const uint32_t* __rest
 }
 
 // ------------------------------------------------------------------ pass A ----
-// Workgroup (column group grp of 16 columns, class c): computes Y[k1][t2] for k1 = c + D*k1',
+// Workgroup (column group grp of CPW columns, class c): computes Y[k1][t2] for k1 = c + D*k1',
 // t2 in the group:  Y = w_N^{t2 k1} * FFT_M( w_N1^{u c} * sum_v z[t2 + N2(u + M v)] w_D^{v c} ).
-// The class sum over v is one or two 256-entry table lookups per operand half (bits of tb).
-// Plan 16 x 32 for M = 512: stage 1 computed in registers from the bit masks, stage 2 written
-// straight to Y (thread t: column t mod 16, butterfly t / 16, rows k1' = j + 16 r).
+// The class sum over v is one or two 256-entry table lookups per operand half (bits of tb).  The
+// class values go to LDS and the M-point FFT runs there.  M = 512 (N1 = 512 D, the 2^27-bit
+// block's shape) runs in k_fwd_cols9p instead.
 template <int LGM, int D, bool BATCH, int CPW = fwd_cols_width(LGM)>
 __global__ __launch_bounds__(FWD_NT) void k_fwd_cols(const uint32_t* __restrict__ tb,
                                                      float2* __restrict__ Y, Geom g,
@@ -176,7 +176,7 @@ __global__ __launch_bounds__(FWD_NT) void k_fwd_cols(const uint32_t* __restrict_
     constexpr int M = 1 << LGM;
     constexpr int NT = FWD_NT;
     constexpr int SEQS = seq_stride(M);
-    static_assert(CPW == 16 || LGM != 9, "the fused M = 512 plan needs 16 columns");
+    static_assert(LGM != 9, "512-point classes run in k_fwd_cols9p");
     __shared__ float2 buf[CPW * SEQS];
     __shared__ float2 twl[TWL_SIZE];
     constexpr int BW = cls_bits(D);         // bits per table: v in [BW q, BW q + BW)
@@ -186,21 +186,19 @@ __global__ __launch_bounds__(FWD_NT) void k_fwd_cols(const uint32_t* __restrict_
     __shared__ float2 TU[M];                // w_N1^{u c}
     const uint32_t logical = xcd_logical(blockIdx.x, gridDim.x);
     const uint32_t c = logical % D, grp = logical / D;  // columns [CPW grp, CPW grp + CPW)
-    auto fill_tables = [&]() {
-        load_twl(twl, tw14, NT);
-        {
-            const float2* src = cls + cls_table_offset(D) + (size_t)c * (NB << BW);
-            for (int e = threadIdx.x; e < NB << BW; e += NT) TB[e >> BW][e & (int)MASK] = src[e];
-        }
-        // w_N1^{u c}: from the 2^14 table while N1 <= 2^14, else from the length-L two-level table
-        if (g.N1 <= (uint32_t)TW14) {
-            const uint32_t step_n1 = TW14 / g.N1;
-            for (int u = threadIdx.x; u < M; u += NT) TU[u] = tw14[((uint32_t)u * c) * step_n1];
-        } else {
-            const uint32_t step_l = g.L / g.N1;
-            for (int u = threadIdx.x; u < M; u += NT) TU[u] = tw_two_level(twlo, twhi, ((uint32_t)u * c) * step_l);
-        }
-    };
+    load_twl(twl, tw14, NT);
+    {
+        const float2* src = cls + cls_table_offset(D) + (size_t)c * (NB << BW);
+        for (int e = threadIdx.x; e < NB << BW; e += NT) TB[e >> BW][e & (int)MASK] = src[e];
+    }
+    // w_N1^{u c}: from the 2^14 table while N1 <= 2^14, else from the length-L two-level table
+    if (g.N1 <= (uint32_t)TW14) {
+        const uint32_t step_n1 = TW14 / g.N1;
+        for (int u = threadIdx.x; u < M; u += NT) TU[u] = tw14[((uint32_t)u * c) * step_n1];
+    } else {
+        const uint32_t step_l = g.L / g.N1;
+        for (int u = threadIdx.x; u < M; u += NT) TU[u] = tw_two_level(twlo, twhi, ((uint32_t)u * c) * step_l);
+    }
     // the element z(col, u) of this class (both halves of tb: even and odd bits)
     auto zval = [&](uint32_t t, int u, auto is_c0) {
         float2 sE = TB[0][t & MASK], sO = TB[0][(t >> 16) & MASK];
@@ -218,77 +216,30 @@ __global__ __launch_bounds__(FWD_NT) void k_fwd_cols(const uint32_t* __restrict_
         }
         return z;
     };
-    if constexpr (LGM == 9) {
-        static_assert(NT == 256, "fused pass A expects 256 threads");
-        const int col = threadIdx.x & 15;
-        const uint32_t t2 = grp * 16u + (uint32_t)col;
-        // stage 1 (radix 16): butterflies j = t/16 + 16 i (i < 2), inputs u = j + 32 r
-        uint32_t tv[2][16];
+    constexpr int IPT = (CPW * M + NT - 1) / NT;
+    uint32_t tbv[IPT];
 #pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            const int j = (threadIdx.x >> 4) + 16 * i;
+    for (int i = 0; i < IPT; ++i) {
+        const int e = threadIdx.x + i * NT;
+        tbv[i] = (e < CPW * M) ? tb[(uint32_t)(e / CPW) * g.N2 + grp * CPW + (uint32_t)(e % CPW)] : 0u;
+    }
+    __syncthreads();
 #pragma unroll
-            for (int r = 0; r < 16; ++r) tv[i][r] = tb[(uint32_t)(j + 32 * r) * g.N2 + t2];
-        }
-        fill_tables();  // its loads go out after the bit words: both latencies overlap
-        __syncthreads();  // tables
-        auto stage1 = [&](auto is_c0) {
-#pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                const int j = (threadIdx.x >> 4) + 16 * i;
-                float2 v[16];
-#pragma unroll
-                for (int r = 0; r < 16; ++r) v[r] = zval(tv[i][r], j + 32 * r, is_c0);
-                stage_first_to_lds<16>(v, buf + col * SEQS, j);
-            }
-        };
-        if (c == 0) stage1(std::true_type{});
-        else stage1(std::false_type{});
-        __syncthreads();
-        // stage 2 (radix 32, last): butterfly j = t/16, outputs k1' = j + 16 r
-        const int j = threadIdx.x >> 4;
-        float2 F[32];
-        stage_last_from_lds<32, M>(F, buf + col * SEQS, j, twl);
-        // w_N^{t2 k1} = w_L^{2 t2 (c + D j)} * w_L^{2 t2 16 D r}
-        const float2 base = tw_two_level(twlo, twhi, (uint32_t)((2ull * t2 * (c + (uint64_t)D * j)) & (g.L - 1u)));
-        const float2 step = tw_two_level(twlo, twhi, (uint32_t)((2ull * t2 * 16ull * D) & (g.L - 1u)));
-        float2 tw = base;
-#pragma unroll
-        for (int r = 0; r < 32; ++r) {
-            if ((r & 7) == 0 && r)  // re-anchor every 8 steps to bound the recurrence error
-                tw = cmul(base, tw_two_level(twlo, twhi, (uint32_t)((2ull * t2 * 16ull * D * r) & (g.L - 1u))));
-            const uint32_t k1 = c + (uint32_t)D * (uint32_t)(j + 16 * r);
-            st_nt(&Y[(size_t)k1 * g.N2 + t2], cmul(F[r], tw));
-            tw = cmul(tw, step);
-        }
-    } else {
-        fill_tables();
-        constexpr int IPT = (CPW * M + NT - 1) / NT;
-        uint32_t tbv[IPT];
-#pragma unroll
-        for (int i = 0; i < IPT; ++i) {
-            const int e = threadIdx.x + i * NT;
-            tbv[i] = (e < CPW * M) ? tb[(uint32_t)(e / CPW) * g.N2 + grp * CPW + (uint32_t)(e % CPW)] : 0u;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < IPT; ++i) {
-            const int e = threadIdx.x + i * NT;
-            if (e < CPW * M)
-                buf[(e % CPW) * SEQS + padi(e / CPW)] =
-                    c == 0 ? zval(tbv[i], e / CPW, std::true_type{}) : zval(tbv[i], e / CPW, std::false_type{});
-        }
-        __syncthreads();
-        lds_fft<LGM, CPW, SEQS, NT>(buf, twl);
-        for (int e = threadIdx.x; e < CPW * M; e += NT) {
-            const int col = e % CPW, k1p = e / CPW;
-            const uint32_t k1 = c + (uint32_t)D * (uint32_t)k1p, t2 = grp * CPW + (uint32_t)col;
-            const float2 tw = tw_two_level(twlo, twhi, (uint32_t)((2ull * t2 * k1) & (g.L - 1u)));
-            st_nt(&Y[(size_t)k1 * g.N2 + t2], cmul(buf[col * SEQS + padi(k1p)], tw));
-        }
+    for (int i = 0; i < IPT; ++i) {
+        const int e = threadIdx.x + i * NT;
+        if (e < CPW * M)
+            buf[(e % CPW) * SEQS + padi(e / CPW)] =
+                c == 0 ? zval(tbv[i], e / CPW, std::true_type{}) : zval(tbv[i], e / CPW, std::false_type{});
+    }
+    __syncthreads();
+    lds_fft<LGM, CPW, SEQS, NT>(buf, twl);
+    for (int e = threadIdx.x; e < CPW * M; e += NT) {
+        const int col = e % CPW, k1p = e / CPW;
+        const uint32_t k1 = c + (uint32_t)D * (uint32_t)k1p, t2 = grp * CPW + (uint32_t)col;
+        const float2 tw = tw_two_level(twlo, twhi, (uint32_t)((2ull * t2 * k1) & (g.L - 1u)));
+        st_nt(&Y[(size_t)k1 * g.N2 + t2], cmul(buf[col * SEQS + padi(k1p)], tw));
     }
 }
-
 
 // Raw buffer access (vector memory instructions): one per-lane byte offset plus a uniform
 // scalar offset, so a run of accesses at fixed strides keeps a single address register.
@@ -1166,16 +1117,14 @@ hipError_t launch_fwd_cols_t(const uint32_t* tb, float2* Y, const Geom& g, const
         const uint32_t groups = g.N2 / 16u, target = Y2 ? 256u : 512u;
         uint32_t per = Y2 ? 16u : 8u;
         while (per > 1u && groups * (uint32_t)D / per < target) per >>= 1;
-        const uint32_t G = groups * (uint32_t)D / per;
-        if ((groups % per) == 0u) {
-            k_fwd_cols9p<D, false><<<dim3(G, Y2 ? 2 : 1), FWD_NT, 0, s>>>(tb, Y, g, t.tw14, t.lo, t.hi, t.cls, tb2, Y2,
-                                                                         per, nullptr, 0);
-            return hipGetLastError();
-        }
+        const uint32_t G = groups * (uint32_t)D / per;  // groups = 512 and per divides it
+        k_fwd_cols9p<D, false><<<dim3(G, Y2 ? 2 : 1), FWD_NT, 0, s>>>(tb, Y, g, t.tw14, t.lo, t.hi, t.cls, tb2, Y2,
+                                                                     per, nullptr, 0);
+    } else {
+        const uint32_t grid = (g.N2 / (uint32_t)fwd_cols_width(LGM)) * (uint32_t)D;
+        k_fwd_cols<LGM, D, false><<<dim3(grid, Y2 ? 2 : 1), FWD_NT, 0, s>>>(tb, Y, g, t.tw14, t.lo, t.hi, t.cls,
+                                                                        nullptr, 0, tb2, Y2);
     }
-    const uint32_t grid = (g.N2 / (uint32_t)fwd_cols_width(LGM)) * (uint32_t)D;
-    k_fwd_cols<LGM, D, false><<<dim3(grid, Y2 ? 2 : 1), FWD_NT, 0, s>>>(tb, Y, g, t.tw14, t.lo, t.hi, t.cls, nullptr, 0,
-                                                                    tb2, Y2);
     return hipGetLastError();
 }
 template <int LG1>
@@ -1332,16 +1281,14 @@ template <int LGM, int D>
 hipError_t batch_fwd_cols_t(const BlockArgs* bt, uint32_t count, const Geom& g0, const Tables& t, int is_seed,
                             hipStream_t s) {
     if constexpr (LGM == 9) {
-        // persistent form, four column groups per workgroup, one class each
+        // four column groups per workgroup, one class each: 128 D workgroups per block
         const uint32_t groups = g0.N2 / 16u, per = 4u, G = groups * (uint32_t)D / per;
-        if ((G % (uint32_t)D) == 0u && (G % 8u) == 0u) {
-            k_fwd_cols9p<D, true><<<dim3(G, 1, count), FWD_NT, 0, s>>>(nullptr, nullptr, g0, t.tw14, t.lo, t.hi, t.cls,
-                                                                      nullptr, nullptr, per, bt, is_seed);
-            return hipGetLastError();
-        }
+        k_fwd_cols9p<D, true><<<dim3(G, 1, count), FWD_NT, 0, s>>>(nullptr, nullptr, g0, t.tw14, t.lo, t.hi, t.cls,
+                                                                  nullptr, nullptr, per, bt, is_seed);
+    } else {
+        k_fwd_cols<LGM, D, true><<<dim3((g0.N2 / (uint32_t)fwd_cols_width(LGM)) * (uint32_t)D, 1, count), FWD_NT, 0,
+                                   s>>>(nullptr, nullptr, g0, t.tw14, t.lo, t.hi, t.cls, bt, is_seed, nullptr, nullptr);
     }
-    k_fwd_cols<LGM, D, true><<<dim3((g0.N2 / (uint32_t)fwd_cols_width(LGM)) * (uint32_t)D, 1, count), FWD_NT, 0, s>>>(
-        nullptr, nullptr, g0, t.tw14, t.lo, t.hi, t.cls, bt, is_seed, nullptr, nullptr);
     return hipGetLastError();
 }
 template <int LG1>
