@@ -92,7 +92,8 @@ std::vector<uint8_t> Endpoint::ready_command(const std::string& socket_type) {
 
 void Endpoint::stop() {
     stop_ = true;
-    if (fd_ >= 0) ::shutdown(fd_, SHUT_RDWR);
+    const int fd = fd_.load();
+    if (fd >= 0) ::shutdown(fd, SHUT_RDWR);  // wakes a blocked transfer; loops also poll stop_
 }
 
 void Endpoint::drop() {

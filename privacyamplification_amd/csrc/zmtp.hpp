@@ -8,6 +8,7 @@
 // One peer per endpoint; a lost connection is re-established and the message retried, as the
 // reference retries failed receives and sends.
 #pragma once
+#include <atomic>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -49,8 +50,9 @@ private:
     std::string role_, host_;
     int port_ = 0;
     bool bind_ = false;
-    int listen_fd_ = -1, fd_ = -1;
-    volatile bool stop_ = false;
+    int listen_fd_ = -1;
+    std::atomic<int> fd_{-1};  // read by stop() from another thread
+    std::atomic<bool> stop_{false};
 };
 
 }  // namespace zmtp
