@@ -633,6 +633,11 @@ int64_t pa_or_amp_out(const uint32_t* key, const uint32_t* seed, uint32_t n, uin
     if (n < 64 || (n % 32) || m == 0 || (m % 32) || m >= n) return -1;
     uint32_t h = n - m, kbits = h + 1;
     uint32_t out_bits = do_compress ? m : n / 2;
+    /* ToBinaryArray XORs the key rest into every word it writes (PrivacyAmplification.cu:753-777,
+     * launched over vertical_block words at :2029); sendData sends out_bits of them (:1186-1197).
+     * Without compression and m > n/2 only the first n/2 output bits exist, so the XOR stops
+     * there; with m < n/2 the key rest ends at bit m. */
+    uint32_t xor_bits = m < out_bits ? m : out_bits;
     uint32_t nw = n / 32;
     uint32_t* ks = (uint32_t*)calloc(nw + 1, 4);
     uint32_t* kr = (uint32_t*)calloc(nw + 1, 4);
@@ -657,7 +662,7 @@ int64_t pa_or_amp_out(const uint32_t* key, const uint32_t* seed, uint32_t n, uin
             uint32_t* krx = (uint32_t*)calloc(nwo, 4);
             if (!words || !krx) rc = -1;
             else {
-                if (do_xor) memcpy(krx, kr, (size_t)(m / 32) * 4);
+                if (do_xor) memcpy(krx, kr, (size_t)(xor_bits / 32) * 4);
                 pa_or_to_binary_array(v, out_bits, norm, corr, do_xor ? krx : NULL, reverse_endian, words);
                 memcpy(out, words, reverse_endian ? (out_bits + 7) / 8 : (size_t)nwo * 4);
             }
@@ -668,7 +673,7 @@ int64_t pa_or_amp_out(const uint32_t* key, const uint32_t* seed, uint32_t n, uin
         rc = pa_or_conv_bits(ks, kbits, seed, n, out_bits, conv, method);
         if (rc == 0) {
             if (do_xor)
-                for (uint32_t p = 0; p < m; ++p)
+                for (uint32_t p = 0; p < xor_bits; ++p)
                     if (refbit(kr, p)) conv[p >> 3] ^= (uint8_t)(0x80u >> (p & 7));
             if (reverse_endian) {
                 memcpy(out, conv, (out_bits + 7) / 8);

@@ -276,7 +276,7 @@ uint64_t workspace_bytes(const Geom& g) {
         b += N * sizeof(float2) + (N + 1) * sizeof(float2);
         b += 2 * (N / g.D) * 4 + pa::nib_bytes(g.N, g.W) + 192;
     }
-    b += 256 + (g.n / 32 + 2) * 4 * 2 + (g.n / 16 + 4);
+    b += 256 + (uint64_t)(g.L / 32 + 2) * 4 * 2 + (g.L / 8 + 16);  // key/seed and output staging
     return b;
 }
 
@@ -314,7 +314,8 @@ int get_workspace(pa_engine* e, const Geom& g, Workspace** out) {
     if (rc != PA_OK) return rc;
     auto w = std::make_unique<Workspace>();
     w->L = g.L;
-    const size_t key_bytes = (size_t)(g.L / 32 + 2) * 4, out_bytes = (size_t)(g.L / 16 + 16);
+    // out: m/8 bytes with compression, n/16 without; any m < n <= L fits in L/8
+    const size_t key_bytes = (size_t)(g.L / 32 + 2) * 4, out_bytes = (size_t)(g.L / 8 + 16);
     struct A { void** p; size_t b; };
     std::vector<A> allocs = {
         {(void**)&w->counts, 2 * pa::POP_SLOTS * sizeof(uint32_t)},

@@ -323,7 +323,8 @@ int main(int argc, char** argv) {
     std::vector<uint8_t*> outs(ocache);
     for (auto& p : keys) rc |= pa_host_alloc(eng, (nw + 2) * 4ull, reinterpret_cast<void**>(&p));
     for (auto& p : seeds) rc |= pa_host_alloc(eng, nw * 4ull, reinterpret_cast<void**>(&p));
-    for (auto& p : outs) rc |= pa_host_alloc(eng, n / 16 + 16, reinterpret_cast<void**>(&p));
+    // m/8 bytes for any vertical_block the key socket accepts (m < n), n/16 without compression
+    for (auto& p : outs) rc |= pa_host_alloc(eng, n / 8 + 16, reinterpret_cast<void**>(&p));
     if (rc != PA_OK) {
         say("Pinned buffer allocation failed => terminating!\n");
         return 1;

@@ -155,6 +155,57 @@ def test_large_non_power_of_two_blocks(engine, oracle, n, m):
     assert all(np.array_equal(o, ref) for o in outs)
 
 
+@pytest.mark.parametrize("e,m_of", [(16, lambda n: n - 32), (20, lambda n: 7 * n // 8), (23, lambda n: 7 * n // 8),
+                                    (23, lambda n: n - 32)])
+def test_output_longer_than_half_block(engine, oracle, e, m_of):
+    """m > n/2: the compressed output (m/8 bytes) is longer than the n/16 bytes of the
+    uncompressed one, and the host-buffer path's staging must hold it."""
+    n = 1 << e
+    m = m_of(n)
+    say(f"long output n=2^{e} m={m}")
+    key = oracle.synth_bits(0xE00 + e, n + 1)
+    seed = oracle.synth_bits(0xF00 + e, n)
+    ref = oracle.amp_out(key, seed, n, m, method=oracle.NTT)
+    assert np.array_equal(engine.amplify(key, seed, n, m), ref)
+    ticket, out, keep = engine.submit(key, seed, n, m)  # asynchronous path, its own staging
+    engine.wait(ticket)
+    assert np.array_equal(out[: m // 8], ref)
+    # without compression only n/2 bits go out, each with its key-rest bit XORed in
+    ref_nc = oracle.amp_out(key, seed, n, m, True, False, method=oracle.NTT)
+    assert np.array_equal(engine.amplify(key, seed, n, m, True, False), ref_nc)
+
+
+def test_random_sweep_vs_oracle(engine, oracle):
+    """40 seeded random blocks across the whole size range, against the NTT oracle:
+    * n log-uniform in 2^10..2^24, half of them non-2^k (compressed, as those require);
+    * m any multiple of 32 below n;
+    * random XOR and compress flags where defined, and some biased keys.
+    Each block runs alone and, all together, in one batch."""
+    rng = np.random.default_rng(0x5EE9)
+    blocks, refs = [], []
+    for i in range(40):
+        e = rng.uniform(10, 24)
+        pow2 = bool(rng.random() < 0.5)
+        n = 1 << int(round(e)) if pow2 else max(64, int(2 ** e) // 32 * 32)
+        m = int(rng.integers(1, n // 32)) * 32
+        do_xor = bool(rng.random() < 0.7)
+        do_compress = True if not pow2 else bool(rng.random() < 0.7)
+        p_key = float(rng.choice([0.5, 0.5, 0.9, 0.1]))
+        key = oracle.synth_bits(0xC00 + i, n + 1, p_key)
+        if not do_xor:
+            key = key[: (n - m) // 32 + 1].copy()
+        seed = oracle.synth_bits(0xD00 + i, n)
+        say(f"sweep {i}: n={n} m={m} xor={do_xor} compress={do_compress} p={p_key}")
+        ref = oracle.amp_out(key, seed, n, m, do_xor, do_compress, method=oracle.NTT)
+        assert np.array_equal(engine.amplify(key, seed, n, m, do_xor, do_compress), ref), i
+        blocks.append({"n": n, "m": m, "key": key, "seed": seed, "do_xor_key_rest": do_xor,
+                       "do_compress": do_compress})
+        refs.append(ref)
+    outs = engine.run_batch(blocks)
+    for i, (o, ref) in enumerate(zip(outs, refs)):
+        assert np.array_equal(o, ref), i
+
+
 def test_async_submit_wait(engine, oracle):
     n, m = 1 << 16, 3 << 13
     say("async submit x4")

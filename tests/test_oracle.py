@@ -120,6 +120,23 @@ def test_methods_agree(oracle, e):
     assert len(set(outs.values())) == 1
 
 
+@pytest.mark.parametrize("n,m", [(512, 384), (1024, 992), (1024, 256)])
+@pytest.mark.parametrize("method", METHODS)
+def test_xor_without_compression_against_brute_force(oracle, n, m, method):
+    """XOR mode without compression sends n/2 output bits (sendData, PrivacyAmplification.cu:
+    1186-1197); ToBinaryArray XORs the key rest into each of them that has one (:753-777): all
+    n/2 when m > n/2, the first m otherwise."""
+    h = n - m
+    key = oracle.synth_bits(0x5A + n + m, n + 1)
+    seed = oracle.synth_bits(0x5B + n + m, n)
+    got = _bits_msb(oracle.amp_out(key, seed, n, m, do_xor=True, do_compress=False,
+                                   method=getattr(oracle, method)), n // 2)
+    conv = _brute(key, h + 1, seed, n, n // 2)
+    rest = [(key[(h + 1 + p) >> 5] >> (31 - ((h + 1 + p) & 31))) & 1 if p < m else 0
+            for p in range(n // 2)]
+    assert got == [a ^ b for a, b in zip(conv, rest)]
+
+
 def test_ntt_counts_exact(oracle):
     """The NTT returns the exact integer convolution (not only its parity)."""
     n, kbits = 1024, 641
