@@ -147,9 +147,13 @@ struct BatchSlot {
     hipEvent_t fork_ev = nullptr, join_ev[FORKS] = {};
     uint8_t* arena = nullptr;                  // grow-only device arena
     uint64_t arena_cap = 0;
-    pa::BlockArgs* htab = nullptr;             // pinned host copy of the block table
+    pa::BlockArgs* htab = nullptr;             // pinned, mapped host copy of the block table
+    uint8_t* htab_dev = nullptr;               // its device-mapped address
     uint32_t htab_cap = 0;
-    hipEvent_t htab_free = nullptr;            // recorded after the last table upload
+    pa::CopyDesc* hdesc = nullptr;             // staging copy descriptors (pinned, mapped)
+    const pa::CopyDesc* hdesc_dev = nullptr;
+    uint32_t hdesc_cap = 0;
+    hipEvent_t htab_free = nullptr;            // recorded after the last read of htab / hdesc
     bool htab_pending = false;
     std::map<std::string, hipGraphExec_t> graphs;
 };
@@ -181,7 +185,8 @@ struct pa_engine {
     BatchSlot batch;                         // pa_batch_device / pa_run_batch
     std::map<std::string, uint32_t> tuned;   // row-pass grid per (L, seed mode, count)
     struct Server* server = nullptr;         // pa_serve_*
-    std::map<uintptr_t, uint64_t> pinned;    // pa_host_alloc ranges: base -> bytes
+    struct PinRange { uint64_t bytes; uint8_t* dev; };
+    std::map<uintptr_t, PinRange> pinned;    // pa_host_alloc ranges: base -> size, mapped address
     uint8_t* split = nullptr;                // sub-block mode: tile inputs/outputs, band accumulator,
     uint64_t split_cap = 0;                  // host-API staging (grow-only)
     std::mutex pin_mu;
@@ -204,6 +209,39 @@ void dev_free(pa_engine* e, void* p, size_t bytes) {
     if (!p) return;
     (void)hipFree(p);
     e->used_bytes -= bytes;
+}
+
+// Pinned host memory that kernels read and write directly over the bus (mapped, coherent): the
+// staging copy kernel moves batch inputs and outputs through it without a DMA command per range.
+hipError_t host_alloc_mapped(void** h, uint8_t** d, size_t bytes) {
+    *h = nullptr;
+    hipError_t r = hipHostMalloc(h, bytes, hipHostMallocMapped | hipHostMallocCoherent);
+    if (r != hipSuccess) {
+        *h = nullptr;
+        return r;
+    }
+    r = hipHostGetDevicePointer(reinterpret_cast<void**>(d), *h, 0);
+    if (r != hipSuccess) {
+        (void)hipHostFree(*h);
+        *h = nullptr;
+    }
+    return r;
+}
+
+// Grows a pinned, mapped host buffer to at least `need` bytes (contents not kept).
+template <typename T>
+hipError_t host_grow(T** h, uint8_t** d, uint64_t* cap, uint64_t need) {
+    if (*cap >= need) return hipSuccess;
+    if (*h) (void)hipHostFree(*h);
+    *h = nullptr;
+    *cap = 0;
+    const uint64_t want = std::max<uint64_t>(need, need + need / 2);
+    void* p = nullptr;
+    hipError_t r = host_alloc_mapped(&p, d, want);
+    if (r != hipSuccess) return r;
+    *h = static_cast<T*>(p);
+    *cap = want;
+    return hipSuccess;
 }
 
 // Streams and events of a batch slot (own_stream: a launch stream of its own, for serving).
@@ -233,6 +271,7 @@ void slot_release(pa_engine* e, BatchSlot* sl) {
     if (sl->htab_free) (void)hipEventDestroy(sl->htab_free);
     if (sl->arena) dev_free(e, sl->arena, sl->arena_cap);
     if (sl->htab) (void)hipHostFree(sl->htab);
+    if (sl->hdesc) (void)hipHostFree(sl->hdesc);
     if (sl->stream) (void)hipStreamDestroy(sl->stream);
     *sl = BatchSlot();
 }
@@ -789,9 +828,14 @@ int ensure_arena(pa_engine* e, BatchSlot& sl, uint64_t bytes, uint64_t limit = 0
     if (sl.arena) dev_free(e, sl.arena, sl.arena_cap);
     sl.arena = nullptr;
     sl.arena_cap = 0;
-    uint64_t cap = align256(bytes + bytes / 8);
+    // half again as much, so that a stream of growing batches reallocates only a few times
+    uint64_t cap = align256(bytes + bytes / 2);
     if (limit && cap > limit) cap = align256(bytes);
     int rc = dev_alloc(e, (void**)&sl.arena, cap);
+    if (rc == PA_ERR_NOMEM && cap > align256(bytes)) {
+        cap = align256(bytes);
+        rc = dev_alloc(e, (void**)&sl.arena, cap);
+    }
     if (rc != PA_OK) return rc;
     sl.arena_cap = cap;
     return PA_OK;
@@ -890,8 +934,10 @@ int batch_tune_group(pa_engine* e, const BatchPlan& p, BatchGroup& gr, const pa:
 }
 
 // Fills and uploads the block table, then launches (tuning / graph replay as asked).
-// `dev` holds the device pointers of key, seed, out and margin of every block.
-int batch_execute(pa_engine* e, BatchSlot& sl, BatchPlan& p, const pa_block* dev, uint32_t bflags, hipStream_t s) {
+// `dev` holds the device pointers of key, seed, out and margin of every block.  `in` ranges are
+// copied with the table before the transforms, `out` ranges after them, one launch each.
+int batch_execute(pa_engine* e, BatchSlot& sl, BatchPlan& p, const pa_block* dev, uint32_t bflags, hipStream_t s,
+                  const std::vector<pa::CopyDesc>* in = nullptr, const std::vector<pa::CopyDesc>* out = nullptr) {
     const uint32_t count = (uint32_t)p.items.size();
     pa::BlockArgs* dtab = reinterpret_cast<pa::BlockArgs*>(sl.arena);
     uint8_t* scratch = sl.arena + p.table_bytes;
@@ -899,12 +945,17 @@ int batch_execute(pa_engine* e, BatchSlot& sl, BatchPlan& p, const pa_block* dev
         PA_HIP(hipEventSynchronize(sl.htab_free));
         sl.htab_pending = false;
     }
-    if (sl.htab_cap < count) {
-        if (sl.htab) (void)hipHostFree(sl.htab);
-        sl.htab = nullptr;
-        sl.htab_cap = 0;
-        PA_HIP(hipHostMalloc((void**)&sl.htab, (size_t)count * sizeof(pa::BlockArgs), hipHostMallocDefault));
-        sl.htab_cap = count;
+    {
+        uint64_t cap = (uint64_t)sl.htab_cap * sizeof(pa::BlockArgs);
+        PA_HIP(host_grow(&sl.htab, &sl.htab_dev, &cap, (uint64_t)count * sizeof(pa::BlockArgs)));
+        sl.htab_cap = (uint32_t)(cap / sizeof(pa::BlockArgs));
+        const size_t nd = 1 + (in ? in->size() : 0) + (out ? out->size() : 0);
+        uint64_t dcap = (uint64_t)sl.hdesc_cap * sizeof(pa::CopyDesc);
+        uint8_t* ddev = nullptr;
+        const bool grow = dcap < nd * sizeof(pa::CopyDesc);
+        PA_HIP(host_grow(&sl.hdesc, &ddev, &dcap, nd * sizeof(pa::CopyDesc)));
+        if (grow) sl.hdesc_dev = reinterpret_cast<const pa::CopyDesc*>(ddev);
+        sl.hdesc_cap = (uint32_t)(dcap / sizeof(pa::CopyDesc));
     }
     uint64_t off = 0;
     for (auto& gr : p.groups) {
@@ -945,38 +996,70 @@ int batch_execute(pa_engine* e, BatchSlot& sl, BatchPlan& p, const pa_block* dev
                                             : std::min(items, std::max<uint32_t>(1u, ((uint32_t)e->cus + cnt - 1u) / cnt));
         }
     }
-    PA_HIP(hipMemcpyAsync(dtab, sl.htab, (size_t)count * sizeof(pa::BlockArgs), hipMemcpyHostToDevice, s));
-    PA_HIP(hipEventRecord(sl.htab_free, s));
-    sl.htab_pending = true;
+    // the table and the input ranges in one copy launch (read from mapped host memory)
+    uint32_t nd = 0;
+    uint64_t longest = (uint64_t)count * sizeof(pa::BlockArgs);
+    sl.hdesc[nd++] = {sl.htab_dev, reinterpret_cast<uint8_t*>(dtab), longest};
+    if (in)
+        for (const pa::CopyDesc& c : *in) {
+            sl.hdesc[nd++] = c;
+            longest = std::max(longest, c.bytes);
+        }
+    PA_HIP(pa::launch_copies(sl.hdesc_dev, nd, longest, s));
+    sl.htab_pending = true;  // from here on htab_free must be recorded before returning
+    auto finish = [&](int rc) -> int {
+        if (rc == PA_OK && out && !out->empty()) {
+            const uint32_t o0 = nd;
+            uint64_t lo = 0;
+            for (const pa::CopyDesc& c : *out) {
+                sl.hdesc[nd++] = c;
+                lo = std::max(lo, c.bytes);
+            }
+            hipError_t r = pa::launch_copies(sl.hdesc_dev + o0, nd - o0, lo, s);
+            if (r != hipSuccess) {
+                set_error(r, "launch_copies", __FILE__, __LINE__);
+                rc = PA_ERR_DEVICE;
+            }
+        }
+        hipError_t r = hipEventRecord(sl.htab_free, s);
+        if (r != hipSuccess && rc == PA_OK) {
+            set_error(r, "hipEventRecord", __FILE__, __LINE__);
+            rc = PA_ERR_DEVICE;
+        }
+        return rc;
+    };
     if (bflags & PA_BATCH_TUNE)
         for (auto& gr : p.groups)
             if (!gr.small && e->tuned.find(tune_key(gr)) == e->tuned.end()) {
                 int rc = batch_tune_group(e, p, gr, dtab, s);
-                if (rc != PA_OK) return rc;
+                if (rc != PA_OK) return finish(rc);
             }
-    if (!(bflags & PA_BATCH_GRAPH)) return batch_launch_all(e, sl, p, dtab, s);
-    std::string key = std::to_string((uintptr_t)sl.arena) + ":";
-    for (auto& gr : p.groups)
-        key += tune_key(gr) + "x" + std::to_string(gr.grid_x) + "y" + std::to_string(gr.tiles_y) + ";";
-    auto f = sl.graphs.find(key);
-    if (f == sl.graphs.end()) {
-        hipGraph_t graph = nullptr;
-        PA_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        int rc = batch_launch_all(e, sl, p, dtab, s);
-        hipError_t ec = hipStreamEndCapture(s, &graph);
-        if (rc != PA_OK) {
-            if (graph) (void)hipGraphDestroy(graph);
-            return rc;
+    if (!(bflags & PA_BATCH_GRAPH)) return finish(batch_launch_all(e, sl, p, dtab, s));
+    auto replay = [&]() -> int {
+        std::string key = std::to_string((uintptr_t)sl.arena) + ":";
+        for (auto& gr : p.groups)
+            key += tune_key(gr) + "x" + std::to_string(gr.grid_x) + "y" + std::to_string(gr.tiles_y) + ";";
+        auto f = sl.graphs.find(key);
+        if (f == sl.graphs.end()) {
+            hipGraph_t graph = nullptr;
+            PA_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            int rc = batch_launch_all(e, sl, p, dtab, s);
+            hipError_t ec = hipStreamEndCapture(s, &graph);
+            if (rc != PA_OK) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return rc;
+            }
+            PA_HIP(ec);
+            hipGraphExec_t exec = nullptr;
+            hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            PA_HIP(ei);
+            f = sl.graphs.emplace(key, exec).first;
         }
-        PA_HIP(ec);
-        hipGraphExec_t exec = nullptr;
-        hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        PA_HIP(ei);
-        f = sl.graphs.emplace(key, exec).first;
-    }
-    PA_HIP(hipGraphLaunch(f->second, s));
-    return PA_OK;
+        PA_HIP(hipGraphLaunch(f->second, s));
+        return PA_OK;
+    };
+    return finish(replay());
 }
 
 }  // namespace
@@ -1153,22 +1236,25 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
-// True if [p, p + bytes) lies inside one pa_host_alloc allocation.
-bool is_pinned(pa_engine* e, const void* p, uint64_t bytes) {
+// The device-mapped address of p if [p, p + bytes) lies inside one pa_host_alloc allocation,
+// else null.
+uint8_t* pinned_dev(pa_engine* e, const void* p, uint64_t bytes) {
     std::lock_guard<std::mutex> lk(e->pin_mu);
     auto it = e->pinned.upper_bound(reinterpret_cast<uintptr_t>(p));
-    if (it == e->pinned.begin()) return false;
+    if (it == e->pinned.begin()) return nullptr;
     --it;
     const uintptr_t a = reinterpret_cast<uintptr_t>(p);
-    return a >= it->first && a + bytes <= it->first + it->second;
+    if (a < it->first || a + bytes > it->first + it->second.bytes) return nullptr;
+    return it->second.dev + (a - it->first);
 }
 
 struct ServeReq {
     pa_block b{};                  // host pointers
-    bool pin_key = false, pin_seed = false, pin_out = false;  // DMA straight from / to the caller
+    bool stage_out = false;        // output through the slot's host staging (caller's not pinned)
     int status = 1;                // 1 while pending, then the block's final status
     Clock::time_point t_submit, t_done;
-    uint64_t dkey = 0, dseed = 0, dout = 0;  // staging offsets (device arena and pinned host)
+    uint64_t dkey = 0, dseed = 0, dout = 0;  // arena offsets
+    uint64_t hout = 0;                       // host staging offset of a staged output
     size_t key_bytes = 0, seed_bytes = 0, out_bytes = 0;
 };
 
@@ -1177,10 +1263,10 @@ struct ServeSlot {
     bool busy = false;
     hipEvent_t done = nullptr;
     std::vector<ServeReq*> reqs;
-    uint8_t* hstage = nullptr;     // pinned host staging: inputs, then outputs
+    uint8_t* hstage = nullptr;     // pinned, mapped host staging for callers' unpinned buffers
+    uint8_t* hstage_dev = nullptr;
     uint64_t hstage_cap = 0;
     uint64_t bytes = 0;            // device bytes this batch holds against the budget
-    uint64_t in_lo = 0, in_hi = 0, out_lo = 0, out_hi = 0;  // staging ranges (arena offsets)
 };
 
 }  // namespace
@@ -1211,7 +1297,7 @@ bool serve_retire(pa_engine* e, Server* sv) {
         const int st = q == hipSuccess ? PA_OK : PA_ERR_DEVICE;
         if (q != hipSuccess) set_error(q, "serving batch", __FILE__, __LINE__);
         for (ServeReq* r : sl.reqs)
-            if (st == PA_OK && !r->pin_out) memcpy(r->b.out, sl.hstage + (r->dout - sl.in_lo), r->out_bytes);
+            if (st == PA_OK && r->stage_out) memcpy(r->b.out, sl.hstage + r->hout, r->out_bytes);
         const auto now = Clock::now();
         {
             std::lock_guard<std::mutex> lk(sv->mu);
@@ -1297,72 +1383,73 @@ bool serve_admit(pa_engine* e, Server* sv, ServeSlot& sl) {
         return true;
     };
     if (rc != PA_OK) return fail_all(rc);
-    // staging after table + scratch: keys, seeds (one input range), then outputs
+    // arena after table + scratch: keys, seeds, then outputs
     uint64_t off = p.table_bytes + p.scratch_bytes;
-    sl.in_lo = off;
     for (size_t k = 0; k < ok.size(); ++k) {
         ServeReq* r = ok[k];
         const Geom& g = p.items[k].g;
-        r->key_bytes = (size_t)(g.n / 32 + 2) * 4;
+        r->key_bytes = (size_t)key_words_for(g) * 4;  // the kernels read no key word beyond these
         r->dkey = off;
-        off += align256(r->key_bytes);
+        off += align256((uint64_t)(g.n / 32 + 2) * 4);
         r->seed_bytes = p.items[k].dyn ? (size_t)(g.n / 32) * 4 : 0;
         r->dseed = off;
         off += align256(r->seed_bytes);
     }
-    sl.in_hi = off;
-    sl.out_lo = off;
     for (size_t k = 0; k < ok.size(); ++k) {
         ok[k]->out_bytes = out_bytes_of(p.items[k].g);
         ok[k]->dout = off;
         off += align256(ok[k]->out_bytes + 16);
     }
-    sl.out_hi = off;
     rc = ensure_arena(e, sl.bs, off, slot_limit);
     if (rc != PA_OK) return fail_all(rc);
-    const uint64_t hneed = sl.out_hi - sl.in_lo;
-    if (sl.hstage_cap < hneed) {
-        if (sl.hstage) (void)hipHostFree(sl.hstage);
-        sl.hstage = nullptr;
-        sl.hstage_cap = 0;
-        if (hipHostMalloc((void**)&sl.hstage, hneed, hipHostMallocDefault) != hipSuccess) return fail_all(PA_ERR_NOMEM);
-        sl.hstage_cap = hneed;
-    }
-    uint8_t* H = sl.hstage - sl.in_lo;  // H + arena offset = pinned address
+    // Every range moves through the staging copy kernel: straight from / to the caller's buffer
+    // when it is pinned (pa_host_alloc), else through the slot's mapped host staging.
     uint8_t* A = sl.bs.arena;
-    std::vector<pa_block> dev(blocks);
-    std::vector<size_t> kws(ok.size());
+    std::vector<const uint8_t*> ksrc(ok.size()), ssrc(ok.size());
+    std::vector<uint8_t*> odst(ok.size());
+    std::vector<uint64_t> hk(ok.size(), UINT64_MAX), hs(ok.size(), UINT64_MAX);
+    uint64_t hneed = 0;
     for (size_t k = 0; k < ok.size(); ++k) {
         ServeReq* r = ok[k];
-        const Geom& g = p.items[k].g;
-        kws[k] = (size_t)key_words_for(g) * 4;  // the kernels read no key word beyond these
-        r->pin_key = is_pinned(e, r->b.key, kws[k]);
-        r->pin_seed = r->seed_bytes && is_pinned(e, r->b.seed, r->seed_bytes);
-        r->pin_out = is_pinned(e, r->b.out, r->out_bytes);
-        if (!r->pin_key) memcpy(H + r->dkey, r->b.key, kws[k]);
-        if (r->seed_bytes && !r->pin_seed) memcpy(H + r->dseed, r->b.seed, r->seed_bytes);
+        ksrc[k] = pinned_dev(e, r->b.key, r->key_bytes);
+        if (!ksrc[k]) { hk[k] = hneed; hneed += align256(r->key_bytes); }
+        if (r->seed_bytes) {
+            ssrc[k] = pinned_dev(e, r->b.seed, r->seed_bytes);
+            if (!ssrc[k]) { hs[k] = hneed; hneed += align256(r->seed_bytes); }
+        }
+        odst[k] = pinned_dev(e, r->b.out, r->out_bytes);
+        r->stage_out = !odst[k];
+        if (r->stage_out) { r->hout = hneed; hneed += align256(r->out_bytes); }
+    }
+    if (hneed && host_grow(&sl.hstage, &sl.hstage_dev, &sl.hstage_cap, hneed) != hipSuccess) return fail_all(PA_ERR_NOMEM);
+    std::vector<pa::CopyDesc> in, outd;
+    in.reserve(2 * ok.size());
+    outd.reserve(ok.size());
+    std::vector<pa_block> dev(blocks);
+    for (size_t k = 0; k < ok.size(); ++k) {
+        ServeReq* r = ok[k];
+        if (!ksrc[k]) {
+            memcpy(sl.hstage + hk[k], r->b.key, r->key_bytes);
+            ksrc[k] = sl.hstage_dev + hk[k];
+        }
+        in.push_back({ksrc[k], A + r->dkey, r->key_bytes});
         dev[k].key = reinterpret_cast<const uint32_t*>(A + r->dkey);
-        if (r->seed_bytes) dev[k].seed = reinterpret_cast<const uint32_t*>(A + r->dseed);
+        if (r->seed_bytes) {
+            if (!ssrc[k]) {
+                memcpy(sl.hstage + hs[k], r->b.seed, r->seed_bytes);
+                ssrc[k] = sl.hstage_dev + hs[k];
+            }
+            in.push_back({ssrc[k], A + r->dseed, r->seed_bytes});
+            dev[k].seed = reinterpret_cast<const uint32_t*>(A + r->dseed);
+        }
+        outd.push_back({A + r->dout, r->stage_out ? sl.hstage_dev + r->hout : odst[k], r->out_bytes});
         dev[k].out = A + r->dout;
         dev[k].margin = nullptr;
     }
     hipStream_t s = sl.bs.stream;
     auto launch = [&]() -> int {
-        for (size_t k = 0; k < ok.size(); ++k) {
-            const ServeReq* r = ok[k];
-            const void* ks = r->pin_key ? (const void*)r->b.key : (const void*)(H + r->dkey);
-            PA_HIP(hipMemcpyAsync(A + r->dkey, ks, kws[k], hipMemcpyHostToDevice, s));
-            if (r->seed_bytes) {
-                const void* ss = r->pin_seed ? (const void*)r->b.seed : (const void*)(H + r->dseed);
-                PA_HIP(hipMemcpyAsync(A + r->dseed, ss, r->seed_bytes, hipMemcpyHostToDevice, s));
-            }
-        }
-        int r2 = batch_execute(e, sl.bs, p, dev.data(), sv->cfg.batch_flags, s);
+        int r2 = batch_execute(e, sl.bs, p, dev.data(), sv->cfg.batch_flags, s, &in, &outd);
         if (r2 != PA_OK) return r2;
-        for (const ServeReq* r : ok) {
-            void* dst = r->pin_out ? (void*)r->b.out : (void*)(H + r->dout);
-            PA_HIP(hipMemcpyAsync(dst, A + r->dout, r->out_bytes, hipMemcpyDeviceToHost, s));
-        }
         PA_HIP(hipEventRecord(sl.done, s));
         return PA_OK;
     };
@@ -1406,13 +1493,14 @@ extern "C" {
 int pa_host_alloc(pa_engine* e, uint64_t bytes, void** ptr) {
     if (!e || !ptr || bytes == 0) return PA_ERR_ARG;
     void* p = nullptr;
-    hipError_t r = hipHostMalloc(&p, bytes, hipHostMallocDefault);
+    uint8_t* d = nullptr;
+    hipError_t r = host_alloc_mapped(&p, &d, bytes);
     if (r != hipSuccess) {
         set_error(r, "hipHostMalloc", __FILE__, __LINE__);
         return PA_ERR_NOMEM;
     }
     std::lock_guard<std::mutex> lk(e->pin_mu);
-    e->pinned[reinterpret_cast<uintptr_t>(p)] = bytes;
+    e->pinned[reinterpret_cast<uintptr_t>(p)] = {bytes, d};
     *ptr = p;
     return PA_OK;
 }

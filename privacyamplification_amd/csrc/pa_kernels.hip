@@ -79,6 +79,48 @@ __global__ __launch_bounds__(256) void k_zero_u32(uint32_t* p, uint32_t count, c
     for (uint32_t i = threadIdx.x; i < count; i += 256) p[i] = 0u;
 }
 
+// ------------------------------------------------------------------ staging copies ----
+// One launch moves every staging range of a batch: workgroup (i, y) copies chunks y, y + grid.y,
+// ... of COPY_CHUNK bytes of range d[i].  Host ranges are pinned, device-mapped memory that the
+// kernel reads or writes over the bus, so a batch costs one launch instead of one DMA command
+// per key, seed and output.  The descriptors themselves may live in mapped host memory.
+constexpr uint32_t COPY_NT = 256;
+constexpr uint64_t COPY_CHUNK = 64u * 1024u;
+__global__ __launch_bounds__(COPY_NT) void k_copies(const CopyDesc* __restrict__ d) {
+    const uint8_t* src = d[blockIdx.x].src;
+    uint8_t* dst = d[blockIdx.x].dst;
+    const uint64_t bytes = d[blockIdx.x].bytes;
+    const uint64_t chunks = (bytes + COPY_CHUNK - 1) / COPY_CHUNK;
+    const uint32_t al = (uint32_t)(((uintptr_t)src | (uintptr_t)dst) & 15u);
+    for (uint64_t ch = blockIdx.y; ch < chunks; ch += gridDim.y) {
+        const uint64_t lo = ch * COPY_CHUNK;
+        const uint32_t len = (uint32_t)(bytes - lo < COPY_CHUNK ? bytes - lo : COPY_CHUNK);
+        uint32_t body = 0;
+        if (al == 0) {  // 16-byte words, four loads in flight per thread
+            const uint4* s4 = reinterpret_cast<const uint4*>(src + lo);
+            uint4* d4 = reinterpret_cast<uint4*>(dst + lo);
+            const uint32_t n16 = len / 16;
+            for (uint32_t i = threadIdx.x; i < n16; i += 4 * COPY_NT) {
+                uint4 v[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (i + k * COPY_NT < n16) v[k] = s4[i + k * COPY_NT];
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (i + k * COPY_NT < n16) d4[i + k * COPY_NT] = v[k];
+            }
+            body = n16 * 16;
+        } else if ((al & 3u) == 0) {
+            const uint32_t* s1 = reinterpret_cast<const uint32_t*>(src + lo);
+            uint32_t* d1 = reinterpret_cast<uint32_t*>(dst + lo);
+            const uint32_t n4 = len / 4;
+            for (uint32_t i = threadIdx.x; i < n4; i += COPY_NT) d1[i] = s1[i];
+            body = n4 * 4;
+        }
+        for (uint32_t i = body + threadIdx.x; i < len; i += COPY_NT) dst[lo + i] = src[lo + i];
+    }
+}
+
 // ------------------------------------------------------------------ bit transpose ----
 // tb[u*N2 + t2] = E | O << 16 with E bit v = x[2(t2 + N2(u + M v))], O bit v = the odd bit.
 // Workgroup (u, part): thread k covers t2 in [part*TB_SPAN + 32k, +32): for each class v it reads
@@ -1154,6 +1196,14 @@ hipError_t launch_small_t(const uint32_t* key, const uint32_t* seed, const float
     return hipGetLastError();
 }
 }  // namespace
+
+hipError_t launch_copies(const CopyDesc* d, uint32_t count, uint64_t max_bytes, hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    const uint64_t chunks = (max_bytes + COPY_CHUNK - 1) / COPY_CHUNK;
+    const uint32_t y = (uint32_t)(chunks < 1 ? 1 : (chunks > 32 ? 32 : chunks));
+    k_copies<<<dim3(count, y), COPY_NT, 0, s>>>(d);
+    return hipGetLastError();
+}
 
 hipError_t launch_zero_u32(uint32_t* p, uint32_t count, hipStream_t s) {
     k_zero_u32<false><<<1, 256, 0, s>>>(p, count, nullptr);

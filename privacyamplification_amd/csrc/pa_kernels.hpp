@@ -51,6 +51,15 @@ __host__ __device__ constexpr int fwd_cols_width(int lgm) { return lgm >= 11 ? 8
 __host__ __device__ constexpr uint64_t nib_bytes(uint32_t N, uint32_t W) { return (W == 1 ? (uint64_t)N : N / 2) + 64; }
 __host__ __device__ constexpr int small_threads(int lgn) { return clamp_threads(2 * (1 << lgn) / 16); }
 
+// A staging range: `bytes` from src to dst, either side device memory or pinned, device-mapped
+// host memory.
+struct CopyDesc {
+    const uint8_t* src;
+    uint8_t* dst;
+    uint64_t bytes;
+};
+// Copies every range of d[0 .. count) in one launch; max_bytes = the longest range.
+hipError_t launch_copies(const CopyDesc* d, uint32_t count, uint64_t max_bytes, hipStream_t s);
 hipError_t launch_zero_u32(uint32_t* p, uint32_t count, hipStream_t s);
 // bits2/tb2 (and tb2/Y2) non-null: the seed is transformed in the same launch (grid.y = 2),
 // its popcount into count + POP_SLOTS.
