@@ -157,9 +157,9 @@ int pa_run_block_split(pa_engine* eng, const uint32_t* key, const uint32_t* seed
                        uint32_t m, uint32_t flags, uint32_t sub_log2, uint8_t* out);
 
 /* Pinned host memory for key, seed and output buffers: the counterpart of the reference's
- * cudaMallocHost'd input and output block caches (PrivacyAmplification.cu:1535-1551).  Blocks
- * whose buffers lie in such memory are copied by DMA straight from and to it; other host memory
- * is staged through the engine's own pinned buffers. */
+ * cudaMallocHost'd input and output block caches (PrivacyAmplification.cu:1535-1551).  The memory
+ * is mapped into the device: blocks whose buffers lie in it are read and written there directly by
+ * the engine's copy kernels; other host memory is staged through the engine's own pinned buffers. */
 int pa_host_alloc(pa_engine* eng, uint64_t bytes, void** ptr);
 int pa_host_free(pa_engine* eng, void* ptr);
 
@@ -183,6 +183,10 @@ int pa_serve_start(pa_engine* eng, const pa_serve_config* cfg);
 /* Queues one block (host pointers, which must stay valid until pa_serve_wait returns for it);
  * returns at once with a ticket.  Resident seeds must not be dropped while blocks use them. */
 int pa_serve_submit(pa_engine* eng, const pa_block* block, pa_ticket* ticket);
+/* Queues count blocks under one lock, tickets[i] for blocks[i]: for a caller that holds several
+ * blocks at once (a filled input cache).  Every block is checked first; on an error nothing is
+ * queued. */
+int pa_serve_submit_many(pa_engine* eng, const pa_block* blocks, uint32_t count, pa_ticket* tickets);
 /* Waits for a block (timeout_ns < 0: forever).  On success *latency_ns (may be NULL) receives the
  * time from pa_serve_submit to the output being written; returns the block's own status. */
 int pa_serve_wait(pa_engine* eng, pa_ticket ticket, int64_t timeout_ns, uint64_t* latency_ns);

@@ -1544,25 +1544,39 @@ int pa_serve_start(pa_engine* e, const pa_serve_config* cfg) {
     return PA_OK;
 }
 
-int pa_serve_submit(pa_engine* e, const pa_block* b, pa_ticket* ticket) {
-    if (!e || !e->server || !b || !ticket || !b->key || !b->out || (!b->seed && !b->seed_id) || b->margin)
-        return PA_ERR_ARG;
+int pa_serve_submit_many(pa_engine* e, const pa_block* blocks, uint32_t count, pa_ticket* tickets) {
+    if (!e || !e->server || !blocks || !tickets) return PA_ERR_ARG;
+    for (uint32_t i = 0; i < count; ++i) {
+        const pa_block* b = blocks + i;
+        if (!b->key || !b->out || (!b->seed && !b->seed_id) || b->margin) return PA_ERR_ARG;
+    }
     Server* sv = e->server;
-    auto* r = new ServeReq();
-    r->b = *b;
-    r->t_submit = Clock::now();
+    std::vector<ServeReq*> rs(count);
+    const auto now = Clock::now();
+    for (uint32_t i = 0; i < count; ++i) {
+        rs[i] = new ServeReq();
+        rs[i]->b = blocks[i];
+        rs[i]->t_submit = now;
+    }
     {
         std::lock_guard<std::mutex> lk(sv->mu);
         if (sv->stop) {
-            delete r;
+            for (ServeReq* r : rs) delete r;
             return PA_ERR_ARG;
         }
-        *ticket = sv->next++;
-        sv->reqs[*ticket] = r;
-        sv->queue.push_back(r);
+        for (uint32_t i = 0; i < count; ++i) {
+            tickets[i] = sv->next++;
+            sv->reqs[tickets[i]] = rs[i];
+            sv->queue.push_back(rs[i]);
+        }
     }
     sv->cv_work.notify_one();
     return PA_OK;
+}
+
+int pa_serve_submit(pa_engine* e, const pa_block* b, pa_ticket* ticket) {
+    if (!b || !ticket) return PA_ERR_ARG;
+    return pa_serve_submit_many(e, b, 1, ticket);
 }
 
 int pa_serve_wait(pa_engine* e, pa_ticket ticket, int64_t timeout_ns, uint64_t* latency_ns) {

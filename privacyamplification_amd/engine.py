@@ -136,6 +136,8 @@ def load_library():
     L.pa_host_free.argtypes = [eng, ctypes.c_void_p]
     L.pa_serve_start.argtypes = [eng, ctypes.POINTER(PaServeConfig)]
     L.pa_serve_submit.argtypes = [eng, ctypes.POINTER(PaBlock), ctypes.POINTER(ctypes.c_uint64)]
+    L.pa_serve_submit_many.argtypes = [eng, ctypes.POINTER(PaBlock), ctypes.c_uint32,
+                                       ctypes.POINTER(ctypes.c_uint64)]
     L.pa_serve_wait.argtypes = [eng, ctypes.c_uint64, ctypes.c_int64, ctypes.POINTER(ctypes.c_uint64)]
     L.pa_serve_stop.argtypes = [eng]
     L.pa_stage_timing.argtypes = [eng, ctypes.c_int]
@@ -151,6 +153,7 @@ def load_library():
                  "pa_run_block", "pa_run_block_seeded", "pa_submit", "pa_wait", "pa_block_device",
                  "pa_block_device_seeded", "pa_synth_bits", "pa_stage_timing", "pa_stage_times",
                  "pa_batch_device", "pa_run_batch", "pa_serve_start", "pa_serve_submit",
+                 "pa_serve_submit_many",
                  "pa_serve_wait", "pa_serve_stop", "pa_host_alloc", "pa_host_free",
                  "pa_block_device_split", "pa_run_block_split"):
         getattr(L, name).restype = ctypes.c_int
@@ -364,6 +367,28 @@ class Engine:
         _check(self._L.pa_serve_submit(self._h, ctypes.byref(b), ctypes.byref(t)), "pa_serve_submit")
         self._serve_keep[t.value] = (k, sd, out)
         return t.value
+
+    def serve_submit_many(self, reqs):
+        """Queues several blocks under one lock.  reqs: (key, n, m, out, seed, sid) tuples, seed
+        None for a resident seed sid; XOR and compression on.  Returns the tickets in order."""
+        arr = (PaBlock * len(reqs))()
+        keep = []
+        for b, (key, n, m, out, seed, sid) in zip(arr, reqs):
+            k, _ = _u32(key)
+            b.key = k.ctypes.data
+            sd = None
+            if seed is not None:
+                sd, _ = _u32(seed)
+                b.seed = sd.ctypes.data
+            b.seed_id, b.n, b.m, b.flags = sid, n, m, DEFAULT_FLAGS
+            assert out.dtype == np.uint8 and out.flags.c_contiguous and out.size >= out_bytes(n, m, True)
+            b.out = out.ctypes.data
+            keep.append((k, sd, out))
+        t = (ctypes.c_uint64 * len(reqs))()
+        _check(self._L.pa_serve_submit_many(self._h, arr, len(reqs), t), "pa_serve_submit_many")
+        for i, kp in enumerate(keep):
+            self._serve_keep[t[i]] = kp
+        return list(t)
 
     def serve_wait(self, ticket, timeout_ns=-1):
         """Returns the block's latency in nanoseconds (submit -> output written); raises on error."""

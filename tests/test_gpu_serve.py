@@ -69,8 +69,9 @@ def test_serve_mixed_stream_equals_alone(engine, oracle):
 
 
 def test_serve_pinned_buffers(engine, oracle):
-    """Pinned (pa_host_alloc) keys, seeds and outputs go by DMA without staging; results unchanged,
-    also when pinned and pageable blocks share a batch."""
+    """Pinned (pa_host_alloc) keys, seeds and outputs are read and written in place by the copy
+    kernel, without staging; results unchanged, also when pinned and pageable blocks share a
+    batch."""
     engine.serve_start(slots=2, max_batch=16)
     try:
         say("serve pinned + pageable")
@@ -91,6 +92,46 @@ def test_serve_pinned_buffers(engine, oracle):
             assert np.array_equal(out, _ref(oracle, np.asarray(key), np.asarray(seed), n, m)), (n, m)
     finally:
         engine.serve_stop()
+
+
+def test_serve_submit_many_any_alignment(engine, oracle):
+    """pa_serve_submit_many queues blocks in one call; keys, seeds and outputs at 16-byte, 4-byte
+    and odd-byte offsets (pinned and pageable) all take the copy kernel's matching path."""
+    n0 = 1 << 16
+    seed0 = oracle.synth_bits(0xB0, n0)
+    sid = engine.seed_put(seed0, n0, n0 // 4)
+    engine.serve_start(slots=2, max_batch=64)
+    try:
+        say("serve_submit_many: 30 blocks, mixed alignment")
+        reqs, meta = [], []
+        for i in range(30):
+            static = i % 3 == 0
+            n = n0 if static else [1 << 15, 40000, 1 << 17, 100000][i % 4]
+            m = n // 4 // 32 * 32
+            key = oracle.synth_bits(0xB100 + i, n + 1)
+            seed = seed0 if static else oracle.synth_bits(0xB200 + i, n)
+            kw, ob = key.size, m // 8
+            shift_k, shift_o = [(0, 0), (1, 4), (1, 1), (0, 3)][i % 4]  # words / bytes
+            pinned = i % 2 == 0
+            kb = engine.host_empty(kw + 2, np.uint32) if pinned else np.zeros(kw + 2, np.uint32)
+            kb[shift_k:shift_k + kw] = key
+            obuf = engine.host_empty(ob + 8) if pinned else np.zeros(ob + 8, np.uint8)
+            out = obuf[shift_o:shift_o + ob]
+            sd = None
+            if not static:
+                sb = engine.host_empty(seed.size + 1, np.uint32) if pinned else np.zeros(seed.size + 1, np.uint32)
+                sb[shift_k:shift_k + seed.size] = seed
+                sd = sb[shift_k:shift_k + seed.size]
+            reqs.append((kb[shift_k:shift_k + kw], n, m, out, sd, 0 if sd is not None else sid))
+            meta.append((out, key, seed, n, m, kb, obuf))
+        tickets = engine.serve_submit_many(reqs)
+        assert len(set(tickets)) == len(tickets)
+        for t, (out, key, seed, n, m, _, _) in zip(tickets, meta):
+            engine.serve_wait(t, 60 * 10**9)
+            assert np.array_equal(out, _ref(oracle, key, seed, n, m)), (n, m)
+    finally:
+        engine.serve_stop()
+        engine.seed_drop(sid)
 
 
 def test_serve_budget_admission(engine, oracle):
