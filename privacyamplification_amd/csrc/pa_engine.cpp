@@ -115,6 +115,7 @@ struct Workspace {
     uint32_t* tbs = nullptr;
     uint8_t* nib = nullptr;    // L/4 bytes
     uint32_t* counts = nullptr;  // POP_SLOTS key popcount partials, then POP_SLOTS seed partials
+    bool counts_zero = false;    // counts are known to be zero (the last block's pack cleared them)
     uint32_t* d_key = nullptr;   // staging for the host-buffer API
     uint32_t* d_seed = nullptr;
     uint32_t* d_out = nullptr;
@@ -435,7 +436,9 @@ int run_device(pa_engine* e, const Geom& g, Workspace* w, const uint32_t* key, c
     }
     uint32_t* key_cnt = w->counts;
     uint32_t* seed_cnt = w->counts + pa::POP_SLOTS;
-    PA_HIP(timed(e, ST_ZERO, s, [&] { return pa::launch_zero_u32(w->counts, 2 * pa::POP_SLOTS, s); }));
+    if (!w->counts_zero)
+        PA_HIP(timed(e, ST_ZERO, s, [&] { return pa::launch_zero_u32(w->counts, 2 * pa::POP_SLOTS, s); }));
+    w->counts_zero = false;  // until this block's pack stage is enqueued
     if (!sspec) {  // key and seed share each launch of the bit transpose and of pass A
         PA_HIP(timed(e, ST_TRANSPOSE_BOTH, s, [&] {
             return pa::launch_transpose(key, w->tbk, key_cnt, g, 0, s, seed, w->tbs);
@@ -450,7 +453,8 @@ int run_device(pa_engine* e, const Geom& g, Workspace* w, const uint32_t* key, c
         PA_HIP(timed(e, ST_ROWS, s, [&] { return pa::launch_rows(pa::ROWS_STATIC, w->Yk, S, g, t, e->cus, s); }));
     }
     PA_HIP(timed(e, ST_INV_COLS, s, [&] { return pa::launch_inv_cols(w->Yk, w->nib, g, key_cnt, ws_dev, t, margin, s); }));
-    PA_HIP(timed(e, ST_PACK, s, [&] { return pa::launch_pack(w->nib, key, out, g, s); }));
+    PA_HIP(timed(e, ST_PACK, s, [&] { return pa::launch_pack(w->nib, key, out, g, s, w->counts); }));
+    w->counts_zero = true;  // the pack stage clears the partials for the next block on the stream
     return PA_OK;
 }
 

@@ -884,11 +884,16 @@ __global__ __launch_bounds__(inv_cols_threads(LG1)) void k_inv_cols(
 // (ToBinaryArray, PrivacyAmplification.cu:757-762).
 // CRUMB: pass A' wrote one 2-bit crumb per column, crumb[t2 * N1 + t1]; the nibble of pair p is
 // crumb[2p] << 2 | crumb[2p + 1].
+// zero_counts (single-block form): the popcount partials, which no later stage of this block
+// reads, are cleared for the next block on the stream.
 template <bool BATCH, bool CRUMB>
 __global__ __launch_bounds__(256) void k_pack(const uint8_t* __restrict__ nibT,
                                               const uint32_t* __restrict__ key,
                                               uint32_t* __restrict__ out, Geom g,
-                                              const BlockArgs* __restrict__ bt) {
+                                              const BlockArgs* __restrict__ bt,
+                                              uint32_t* __restrict__ zero_counts) {
+    if (!BATCH && zero_counts && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 2 * POP_SLOTS)
+        zero_counts[threadIdx.x] = 0u;
     __shared__ uint8_t tile[PACK_T1][PACK_G + 4];
     if constexpr (BATCH) {  // grid.y covers the largest window of the launch; rows past this
         const BlockArgs& a = bt[blockIdx.z];  // block's window load nothing and store nothing
@@ -1278,13 +1283,14 @@ hipError_t launch_inv_cols(const float2* R, uint8_t* nib, const Geom& g, const u
     }
 }
 
-hipError_t launch_pack(const uint8_t* nibT, const uint32_t* key, uint32_t* out, const Geom& g, hipStream_t s) {
+hipError_t launch_pack(const uint8_t* nibT, const uint32_t* key, uint32_t* out, const Geom& g, hipStream_t s,
+                       uint32_t* zero_counts) {
     const uint32_t P2 = g.N2 >> 1;
     const uint32_t q_lo = g.out_off >> 2, q_hi = (g.out_off + g.out_bits) >> 2;
     const uint32_t rows = (q_hi - 1u) / P2 - ((q_lo / P2) & ~3u) + 1u;
     dim3 grid(P2 / PACK_G, (rows + PACK_T1 - 1u) / PACK_T1);
-    if (g.W == 1) k_pack<false, true><<<grid, 256, 0, s>>>(nibT, key, out, g, nullptr);
-    else k_pack<false, false><<<grid, 256, 0, s>>>(nibT, key, out, g, nullptr);
+    if (g.W == 1) k_pack<false, true><<<grid, 256, 0, s>>>(nibT, key, out, g, nullptr, zero_counts);
+    else k_pack<false, false><<<grid, 256, 0, s>>>(nibT, key, out, g, nullptr, zero_counts);
     return hipGetLastError();
 }
 
@@ -1424,8 +1430,8 @@ hipError_t launch_batch_inv_cols(const BlockArgs* bt, uint32_t count, const Geom
 
 hipError_t launch_batch_pack(const BlockArgs* bt, uint32_t count, const Geom& g0, uint32_t tiles_y, hipStream_t s) {
     const dim3 grid((g0.N2 >> 1) / PACK_G, tiles_y, count);
-    if (g0.W == 1) k_pack<true, true><<<grid, 256, 0, s>>>(nullptr, nullptr, nullptr, g0, bt);
-    else k_pack<true, false><<<grid, 256, 0, s>>>(nullptr, nullptr, nullptr, g0, bt);
+    if (g0.W == 1) k_pack<true, true><<<grid, 256, 0, s>>>(nullptr, nullptr, nullptr, g0, bt, nullptr);
+    else k_pack<true, false><<<grid, 256, 0, s>>>(nullptr, nullptr, nullptr, g0, bt, nullptr);
     return hipGetLastError();
 }
 

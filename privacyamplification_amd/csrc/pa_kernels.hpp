@@ -74,7 +74,9 @@ hipError_t launch_fwd_cols(const uint32_t* tb, float2* Y, const Geom& g, const T
 hipError_t launch_rows(int mode, float2* Yk, float2* S, const Geom& g, const Tables& t, int cus, hipStream_t s);
 hipError_t launch_inv_cols(const float2* R, uint8_t* nib, const Geom& g, const uint32_t* wk,
                            const uint32_t* ws, const Tables& t, float* margin, hipStream_t s);
-hipError_t launch_pack(const uint8_t* nib, const uint32_t* key, uint32_t* out, const Geom& g, hipStream_t s);
+// zero_counts != null: the launch also clears those 2 * POP_SLOTS popcount partials.
+hipError_t launch_pack(const uint8_t* nib, const uint32_t* key, uint32_t* out, const Geom& g, hipStream_t s,
+                       uint32_t* zero_counts = nullptr);
 hipError_t launch_small(const uint32_t* key, const uint32_t* seed, const float2* sspec,
                         const uint32_t* ws_ptr, uint32_t* out, const Geom& g, const Tables& t,
                         float* margin, float2* sspec_out, uint32_t* ws_out, hipStream_t s);
