@@ -175,6 +175,42 @@ def test_output_longer_than_half_block(engine, oracle, e, m_of):
     assert np.array_equal(engine.amplify(key, seed, n, m, True, False), ref_nc)
 
 
+def _edge_cases():
+    out = []
+    for n in (64, 96, 128, 160, 192, 256, 288, 512, 544, 16384, 16416, 32800):
+        pow2 = n & (n - 1) == 0
+        for m in sorted({32, n // 2 // 32 * 32, n - 32}):
+            for do_xor in (True, False):
+                for do_compress in ((True, False) if pow2 else (True,)):
+                    out.append((n, m, do_xor, do_compress))
+    return out
+
+
+@pytest.mark.parametrize("n,m,do_xor,do_compress", _edge_cases())
+def test_smallest_blocks_and_path_boundaries(engine, oracle, n, m, do_xor, do_compress):
+    """The smallest n (64) and m (32), m = n - 32, non-2^k sizes just above a power of two, and the
+    sizes either side of the one-workgroup path's limit (L = 2^14 / 2^15)."""
+    say(f"edge n={n} m={m} xor={do_xor} compress={do_compress}")
+    key = oracle.synth_bits(0x1E00 + n + m, n + 1)
+    if not do_xor:
+        key = key[: (n - m) // 32 + 1].copy()
+    seed = oracle.synth_bits(0x1F00 + n + m, n)
+    ref = oracle.amp_out(key, seed, n, m, do_xor, do_compress, method=oracle.NTT)
+    assert np.array_equal(engine.amplify(key, seed, n, m, do_xor, do_compress), ref)
+
+
+@pytest.mark.parametrize("m_of", [lambda n: 32, lambda n: n - 32], ids=["m=32", "m=n-32"])
+def test_largest_block_extreme_ratios(engine, oracle, m_of):
+    """n = 2^27 with the shortest and the longest output."""
+    n = 1 << 27
+    m = m_of(n)
+    say(f"extreme ratio n=2^27 m={m}")
+    key = oracle.synth_bits(0x2E00 + m % 997, n + 1)
+    seed = oracle.synth_bits(0x2F00 + m % 997, n)
+    ref = oracle.amp_out(key, seed, n, m, method=oracle.NTT)
+    assert np.array_equal(engine.amplify(key, seed, n, m), ref)
+
+
 def test_random_sweep_vs_oracle(engine, oracle):
     """40 seeded random blocks across the whole size range, against the NTT oracle:
     * n log-uniform in 2^10..2^24, half of them non-2^k (compressed, as those require);
