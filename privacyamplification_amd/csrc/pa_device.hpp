@@ -53,6 +53,36 @@ __device__ __forceinline__ float2 w32pow(float2 v, int r) {
     return make_float2(-t.x, -t.y);
 }
 
+// tan(pi q / 16), q = 1, 2, 3, 4 (exact values rounded to float).
+__host__ __device__ constexpr float tan16(int q) {
+    return q == 1 ? 1.989123674e-01f : q == 2 ? 4.142135624e-01f : q == 3 ? 6.681786379e-01f : 1.0f;
+}
+// For w = exp(-2 pi i k / 32) = c + i s, k in [1, 16): the ratio of magnitude <= 1, s / c where
+// |c| >= |s| (k <= 4 or k >= 12), else c / s.
+__host__ __device__ constexpr bool w32_cform(int k) { return k <= 4 || k >= 12; }
+__host__ __device__ constexpr float w32_ratio(int k) {
+    return k <= 4 ? -tan16(k) : k < 8 ? -tan16(8 - k) : k < 12 ? tan16(k - 8) : tan16(16 - k);
+}
+
+// The butterfly (e + w o, e - w o), w = exp(-2 pi i k / 32) for a compile-time-foldable k in
+// [1, 16) other than 8, in six FMAs instead of a complex product and four additions:
+// w o = c (o + i t o) with t = s / c, or s (u o + i o) with u = c / s, whichever ratio is at
+// most 1 in magnitude (the tangent form of the twiddled butterfly).
+__device__ __forceinline__ void bfly_w32(float2 e, float2 o, int k, float2& lo, float2& hi) {
+    const float r = w32_ratio(k);
+    float2 z;
+    float m;
+    if (w32_cform(k)) {
+        z = make_float2(fmaf(-r, o.y, o.x), fmaf(r, o.x, o.y));
+        m = w32_re(k);
+    } else {
+        z = make_float2(fmaf(r, o.x, -o.y), fmaf(r, o.y, o.x));
+        m = w32_im(k);
+    }
+    lo = make_float2(fmaf(m, z.x, e.x), fmaf(m, z.y, e.y));
+    hi = make_float2(fmaf(-m, z.x, e.x), fmaf(-m, z.y, e.y));
+}
+
 // In-register forward DFT, natural-order input and output (radix-2 decimation in time).
 template <int R>
 __device__ __forceinline__ void dft(float2 (&x)[R]) {
@@ -79,9 +109,14 @@ __device__ __forceinline__ void dft(float2 (&x)[R]) {
         dft<Q>(o);
 #pragma unroll
         for (int k = 0; k < Q; ++k) {
-            const float2 t = twc<R>(o[k], k);
-            x[k] = cadd(e[k], t);
-            x[k + Q] = csub(e[k], t);
+            const int k32 = k * (32 / R);
+            if (k32 == 0 || k32 == 8) {
+                const float2 t = twc<R>(o[k], k);
+                x[k] = cadd(e[k], t);
+                x[k + Q] = csub(e[k], t);
+            } else {
+                bfly_w32(e[k], o[k], k32, x[k], x[k + Q]);
+            }
         }
     }
 }
