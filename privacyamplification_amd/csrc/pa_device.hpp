@@ -287,6 +287,25 @@ __device__ __forceinline__ void stage_last_from_lds(float2 (&v)[R], const float2
     dft<R>(v);
 }
 
+// stage_last_from_lds with the twiddles of butterfly j read from a stage table
+// (fill_stage_table<R, L / R>): trow = table + (j % (L / R)) * (R + 1).
+template <int R, int L>
+__device__ __forceinline__ void stage_last_from_lds_tab(float2 (&v)[R], const float2* seqbuf, int j,
+                                                        const float2* trow) {
+    constexpr int NS = L / R;
+    if constexpr (NS % (1 << PAD_SHIFT) == 0) {
+        const float2* rb = seqbuf + padi(j);
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = rb[r * padi(NS)];
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = seqbuf[padi(j + r * NS)];
+    }
+#pragma unroll
+    for (int r = 1; r < R; ++r) v[r] = cmul(v[r], trow[r]);
+    dft<R>(v);
+}
+
 // Radix-16 stages from span NS up to 2^LG; a final radix-32 stage absorbs a leftover factor 2.
 template <int NS, int LG, int NSEQ, int SEQS, int NT>
 __device__ __forceinline__ void stockham_rest(float2* buf, const float2* twl) {

@@ -359,13 +359,14 @@ __device__ __forceinline__ void fwd9_stage1(const uint32_t (&tv)[2][16], float2*
         stage_first_to_lds<16>(v, buf + col * SEQS, j);
     }
 }
+// tw2: the radix-32 stage's twiddle table, w_512^(jt r) at tw2[jt * 33 + r] (fill_stage_table<32, 16>).
 template <int D>
-__device__ __forceinline__ void fwd9_stage2(const float2* buf, const float2* twl, const float2 (&tw)[10],
+__device__ __forceinline__ void fwd9_stage2(const float2* buf, const float2* tw2, const float2 (&tw)[10],
                                             __amdgpu_buffer_rsrc_t ry, uint32_t N2, uint32_t c, uint32_t t2,
                                             int col, int jt) {
     constexpr int SEQS = seq_stride(512);
     float2 F[32];
-    stage_last_from_lds<32, 512>(F, buf + col * SEQS, jt, twl);
+    stage_last_from_lds_tab<32, 512>(F, buf + col * SEQS, jt, tw2 + jt * 33);
     const float2 base = cmul(tw[0], tw[1]), step = cmul(tw[2], tw[3]);
     const uint32_t voff = ((c + (uint32_t)D * (uint32_t)jt) * N2 + t2) * 8u;
     float2 w = base;
@@ -379,7 +380,7 @@ __device__ __forceinline__ void fwd9_stage2(const float2* buf, const float2* twl
 template <int D, bool C0>
 __device__ __forceinline__ void fwd9_walk(__amdgpu_buffer_rsrc_t rtb, __amdgpu_buffer_rsrc_t ry, const Geom& g,
                                           const float2* __restrict__ twlo, const float2* __restrict__ twhi,
-                                          float2* buf, const float2* twl, const float2 (*TB)[1 << cls_bits(D)],
+                                          float2* buf, const float2* tw2, const float2 (*TB)[1 << cls_bits(D)],
                                           const float2* TU, uint32_t c, uint32_t grp0, uint32_t per) {
     const int col = threadIdx.x & 15;
     const int jt = threadIdx.x >> 4;
@@ -398,7 +399,7 @@ __device__ __forceinline__ void fwd9_walk(__amdgpu_buffer_rsrc_t rtb, __amdgpu_b
         fwd9_load_tv(tv, rtb, N2, (grp0 + 1u) * 16u + (uint32_t)col, jt);
         fwd9_load_tw<D>(twn, twlo, twhi, g.L, c, (grp0 + 1u) * 16u + (uint32_t)col, jt);
     }
-    fwd9_stage2<D>(buf, twl, tw, ry, N2, c, grp0 * 16u + (uint32_t)col, col, jt);
+    fwd9_stage2<D>(buf, tw2, tw, ry, N2, c, grp0 * 16u + (uint32_t)col, col, jt);
     for (uint32_t it = 1; it < per; ++it) {
         const uint32_t grp = grp0 + it;
 #pragma unroll
@@ -410,7 +411,7 @@ __device__ __forceinline__ void fwd9_walk(__amdgpu_buffer_rsrc_t rtb, __amdgpu_b
             fwd9_load_tv(tv, rtb, N2, (grp + 1u) * 16u + (uint32_t)col, jt);
             fwd9_load_tw<D>(twn, twlo, twhi, g.L, c, (grp + 1u) * 16u + (uint32_t)col, jt);
         }
-        fwd9_stage2<D>(buf, twl, tw, ry, N2, c, grp * 16u + (uint32_t)col, col, jt);
+        fwd9_stage2<D>(buf, tw2, tw, ry, N2, c, grp * 16u + (uint32_t)col, col, jt);
     }
 }
 
@@ -444,7 +445,7 @@ __global__ __launch_bounds__(FWD_NT, 2) void k_fwd_cols9p(const uint32_t* __rest
     constexpr int SEQS = seq_stride(M);
     static_assert(NT == 256, "fused pass A expects 256 threads");
     __shared__ float2 buf[16 * SEQS];
-    __shared__ float2 twl[TWL_SIZE];
+    __shared__ float2 tw2[16 * 33];  // radix-32 stage twiddles w_512^(jt r), rows padded to 33
     constexpr int BW = cls_bits(D);
     constexpr int NB = cls_tables(D);
     constexpr uint32_t MASK = (1u << BW) - 1u;
@@ -454,7 +455,7 @@ __global__ __launch_bounds__(FWD_NT, 2) void k_fwd_cols9p(const uint32_t* __rest
     const uint32_t c = logical % D, grp0 = (logical / D) * per;
     const __amdgpu_buffer_rsrc_t rtb = make_rsrc(tb, (uint32_t)M * g.N2 * 4u);
     const __amdgpu_buffer_rsrc_t ry = make_rsrc(Y, g.N1 * g.N2 * 8u);
-    load_twl(twl, tw14, NT);
+    fill_stage_table<32, 16>(tw2, tw14, NT);
     {
         const float2* src = cls + cls_table_offset(D) + (size_t)c * (NB << BW);
         for (int e = threadIdx.x; e < NB << BW; e += NT) TB[e >> BW][e & (int)MASK] = src[e];
@@ -466,8 +467,8 @@ __global__ __launch_bounds__(FWD_NT, 2) void k_fwd_cols9p(const uint32_t* __rest
         const uint32_t step_l = g.L / g.N1;
         for (int u = threadIdx.x; u < M; u += NT) TU[u] = tw_two_level(twlo, twhi, ((uint32_t)u * c) * step_l);
     }
-    if (c == 0) fwd9_walk<D, true>(rtb, ry, g, twlo, twhi, buf, twl, TB, TU, c, grp0, per);
-    else fwd9_walk<D, false>(rtb, ry, g, twlo, twhi, buf, twl, TB, TU, c, grp0, per);
+    if (c == 0) fwd9_walk<D, true>(rtb, ry, g, twlo, twhi, buf, tw2, TB, TU, c, grp0, per);
+    else fwd9_walk<D, false>(rtb, ry, g, twlo, twhi, buf, tw2, TB, TU, c, grp0, per);
 }
 
 // ------------------------------------------------------------------ pass B ----
