@@ -122,6 +122,22 @@ __global__ __launch_bounds__(COPY_NT) void k_copies(const CopyDesc* __restrict__
 }
 
 // ------------------------------------------------------------------ bit transpose ----
+// One round of the recursive 32 x 32 bit-matrix transpose (rows A[k], column 0 = the MSB): swap
+// the off-diagonal J x J blocks of every 2J x 2J block.  Five rounds J = 16, 8, 4, 2, 1 transpose.
+template <int J>
+__device__ __forceinline__ void bit_transpose_round(uint32_t (&A)[32]) {
+    constexpr uint32_t M = J == 16 ? 0x0000FFFFu : J == 8 ? 0x00FF00FFu : J == 4 ? 0x0F0F0F0Fu
+                         : J == 2 ? 0x33333333u : 0x55555555u;
+#pragma unroll
+    for (int b = 0; b < 32; b += 2 * J)
+#pragma unroll
+        for (int r = 0; r < J; ++r) {
+            const uint32_t t = (A[b + r] ^ (A[b + r + J] >> J)) & M;
+            A[b + r] ^= t;
+            A[b + r + J] ^= t << J;
+        }
+}
+
 // tb[u*N2 + t2] = E | O << 16 with E bit v = x[2(t2 + N2(u + M v))], O bit v = the odd bit.
 // Workgroup (u, part): thread k covers t2 in [part*TB_SPAN + 32k, +32): for each class v it reads
 // the two words holding those 64 bits (coalesced across the workgroup), transposes the D x 32
@@ -163,17 +179,39 @@ __global__ __launch_bounds__(TB_NT) void k_transpose_bits(const uint32_t* __rest
         pc += (uint32_t)(__popc(w0[v]) + __popc(w1[v]));
     }
     uint32_t* srow = stage + threadIdx.x * 33;
+    if constexpr (D >= 8) {
+        // Rows A[15 - v] = w1[v], A[31 - v] = w0[v] (zero for v >= D), transposed: A[c] then holds
+        // bit 31 - c of w0[v] at bit v and of w1[v] at bit v + 16, so output i (E bits from the
+        // pair's even bit 2i, MSB first, O bits from the odd one) is two half-words.
+        uint32_t A[32];
 #pragma unroll
-    for (int i = 0; i < 32; ++i) {
-        const int sh = 31 - 2 * (i & 15);  // even bit 2i of the 64-bit pair, MSB first
-        uint32_t E = 0, O = 0;
-#pragma unroll
-        for (int v = 0; v < D; ++v) {
-            const uint32_t w = i < 16 ? w0[v] : w1[v];
-            E |= ((w >> sh) & 1u) << v;
-            O |= ((w >> (sh - 1)) & 1u) << v;
+        for (int v = 0; v < 16; ++v) {
+            A[15 - v] = v < D ? w1[v < D ? v : 0] : 0u;
+            A[31 - v] = v < D ? w0[v < D ? v : 0] : 0u;
         }
-        srow[i] = E | (O << 16);
+        bit_transpose_round<16>(A);
+        bit_transpose_round<8>(A);
+        bit_transpose_round<4>(A);
+        bit_transpose_round<2>(A);
+        bit_transpose_round<1>(A);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            srow[i] = (A[2 * i] & 0xFFFFu) | (A[2 * i + 1] << 16);
+            srow[16 + i] = (A[2 * i] >> 16) | (A[2 * i + 1] & 0xFFFF0000u);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 32; ++i) {
+            const int sh = 31 - 2 * (i & 15);  // even bit 2i of the 64-bit pair, MSB first
+            uint32_t E = 0, O = 0;
+#pragma unroll
+            for (int v = 0; v < D; ++v) {
+                const uint32_t w = i < 16 ? w0[v] : w1[v];
+                E |= ((w >> sh) & 1u) << v;
+                O |= ((w >> (sh - 1)) & 1u) << v;
+            }
+            srow[i] = E | (O << 16);
+        }
     }
     for (int off = 32; off > 0; off >>= 1) pc += __shfl_xor(pc, off);
     if ((threadIdx.x & 63u) == 0u) red[threadIdx.x >> 6] = pc;
