@@ -1240,6 +1240,9 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
+// Serving ranges of at least this many bytes go by DMA, smaller ones through the copy kernel.
+constexpr uint64_t SERVE_DMA_MIN = 1ull << 20;
+
 // The device-mapped address of p if [p, p + bytes) lies inside one pa_host_alloc allocation,
 // else null.
 uint8_t* pinned_dev(pa_engine* e, const void* p, uint64_t bytes) {
@@ -1406,8 +1409,11 @@ bool serve_admit(pa_engine* e, Server* sv, ServeSlot& sl) {
     }
     rc = ensure_arena(e, sl.bs, off, slot_limit);
     if (rc != PA_OK) return fail_all(rc);
-    // Every range moves through the staging copy kernel: straight from / to the caller's buffer
-    // when it is pinned (pa_host_alloc), else through the slot's mapped host staging.
+    // Ranges below SERVE_DMA_MIN move through the staging copy kernel, larger ones by DMA: one
+    // kernel launch replaces thousands of small DMA commands, while the DMA engines move large
+    // blocks alongside the other slots' kernels (the copy kernel cannot: the transform kernels
+    // fill every compute unit's registers).  Either way the host side is the caller's buffer when
+    // it is pinned (pa_host_alloc), else the slot's mapped host staging.
     uint8_t* A = sl.bs.arena;
     std::vector<const uint8_t*> ksrc(ok.size()), ssrc(ok.size());
     std::vector<uint8_t*> odst(ok.size());
@@ -1426,34 +1432,48 @@ bool serve_admit(pa_engine* e, Server* sv, ServeSlot& sl) {
         if (r->stage_out) { r->hout = hneed; hneed += align256(r->out_bytes); }
     }
     if (hneed && host_grow(&sl.hstage, &sl.hstage_dev, &sl.hstage_cap, hneed) != hipSuccess) return fail_all(PA_ERR_NOMEM);
-    std::vector<pa::CopyDesc> in, outd;
+    // kernel ranges use device-mapped host addresses, DMA ranges the host addresses
+    std::vector<pa::CopyDesc> in, outd, dma_in, dma_out;
     in.reserve(2 * ok.size());
     outd.reserve(ok.size());
     std::vector<pa_block> dev(blocks);
+    auto add_in = [&](const void* host, const uint8_t* mapped, uint8_t* d, uint64_t bytes) {
+        if (bytes >= SERVE_DMA_MIN) dma_in.push_back({static_cast<const uint8_t*>(host), d, bytes});
+        else in.push_back({mapped, d, bytes});
+    };
     for (size_t k = 0; k < ok.size(); ++k) {
         ServeReq* r = ok[k];
+        const void* kh = r->b.key;
         if (!ksrc[k]) {
             memcpy(sl.hstage + hk[k], r->b.key, r->key_bytes);
+            kh = sl.hstage + hk[k];
             ksrc[k] = sl.hstage_dev + hk[k];
         }
-        in.push_back({ksrc[k], A + r->dkey, r->key_bytes});
+        add_in(kh, ksrc[k], A + r->dkey, r->key_bytes);
         dev[k].key = reinterpret_cast<const uint32_t*>(A + r->dkey);
         if (r->seed_bytes) {
+            const void* sh = r->b.seed;
             if (!ssrc[k]) {
                 memcpy(sl.hstage + hs[k], r->b.seed, r->seed_bytes);
+                sh = sl.hstage + hs[k];
                 ssrc[k] = sl.hstage_dev + hs[k];
             }
-            in.push_back({ssrc[k], A + r->dseed, r->seed_bytes});
+            add_in(sh, ssrc[k], A + r->dseed, r->seed_bytes);
             dev[k].seed = reinterpret_cast<const uint32_t*>(A + r->dseed);
         }
-        outd.push_back({A + r->dout, r->stage_out ? sl.hstage_dev + r->hout : odst[k], r->out_bytes});
+        if (r->out_bytes >= SERVE_DMA_MIN)
+            dma_out.push_back({A + r->dout, r->stage_out ? sl.hstage + r->hout : r->b.out, r->out_bytes});
+        else
+            outd.push_back({A + r->dout, r->stage_out ? sl.hstage_dev + r->hout : odst[k], r->out_bytes});
         dev[k].out = A + r->dout;
         dev[k].margin = nullptr;
     }
     hipStream_t s = sl.bs.stream;
     auto launch = [&]() -> int {
+        for (const pa::CopyDesc& c : dma_in) PA_HIP(hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyHostToDevice, s));
         int r2 = batch_execute(e, sl.bs, p, dev.data(), sv->cfg.batch_flags, s, &in, &outd);
         if (r2 != PA_OK) return r2;
+        for (const pa::CopyDesc& c : dma_out) PA_HIP(hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyDeviceToHost, s));
         PA_HIP(hipEventRecord(sl.done, s));
         return PA_OK;
     };

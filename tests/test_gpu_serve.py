@@ -134,6 +134,38 @@ def test_serve_submit_many_any_alignment(engine, oracle):
         engine.seed_drop(sid)
 
 
+def test_serve_large_blocks_by_dma(engine, oracle):
+    """Keys, seeds and outputs of 1 MiB or more go by DMA instead of the copy kernel; mixed with
+    small ones, pinned and pageable, in one stream of batches."""
+    cases = [(1 << 23, 3 << 20, True), (1 << 23, 7 << 20, False), (1 << 25, 7 << 22, True),
+             (1 << 25, 3 << 22, False), (1 << 16, 1 << 14, True), (1 << 23, 3 << 20, False)]
+    n0, m0 = 1 << 23, 3 << 20
+    seed0 = oracle.synth_bits(0xD0, n0)
+    sid = engine.seed_put(seed0, n0, m0)
+    engine.serve_start(slots=3, max_batch=4, hbm_budget_bytes=48 << 30)
+    try:
+        say("serve large blocks by DMA")
+        reqs, meta = [], []
+        for i, (n, m, pinned) in enumerate(cases):
+            key = oracle.synth_bits(0xD100 + i, n + 1)
+            resident = i == 5
+            seed = seed0 if resident else oracle.synth_bits(0xD200 + i, n)
+            k = engine.pinned_copy(key) if pinned else key
+            sd = None if resident else (engine.pinned_copy(seed) if pinned else seed)
+            out = engine.host_empty(m // 8) if pinned else np.zeros(m // 8, np.uint8)
+            reqs.append((k, n, m, out, sd, sid if resident else 0))
+            meta.append((out, key, seed, n, m))
+        tickets = engine.serve_submit_many(reqs)
+        for t, (out, key, seed, n, m) in zip(tickets, meta):
+            engine.serve_wait(t, 120 * 10**9)
+            assert np.array_equal(out, engine.amplify(key, seed, n, m)), (n, m)
+            if n <= 1 << 23:
+                assert np.array_equal(out, _ref(oracle, key, seed, n, m)), (n, m)
+    finally:
+        engine.serve_stop()
+        engine.seed_drop(sid)
+
+
 def test_serve_budget_admission(engine, oracle):
     """A small budget splits the stream into small batches; a block larger than a slot's share of
     the budget is refused with PA_ERR_NOMEM and the others still complete."""
