@@ -164,6 +164,8 @@ struct BatchSlot {
 struct pa_engine {
     pa_config cfg{};
     hipStream_t stream = nullptr;
+    hipStream_t upload = nullptr;        // pa_run_block: a fresh seed's host-to-device copy
+    hipEvent_t upload_done = nullptr, key_up = nullptr;
     float2* tw14 = nullptr;
     float2* cls = nullptr;  // pass-A class tables (pa::CLS_TABLE_TOTAL entries)
     int cus = 256;  // compute units of the device (persistent kernels launch one workgroup each)
@@ -393,10 +395,23 @@ pa::Tables tables(pa_engine* e, uint32_t L) {
 hipStream_t stream_of(pa_engine* e, void* s) { return s ? (hipStream_t)s : e->stream; }
 
 enum Stage { ST_SMALL, ST_ZERO, ST_TRANSPOSE_KEY, ST_FWD_COLS_KEY, ST_TRANSPOSE_BOTH, ST_FWD_COLS_BOTH,
-             ST_ROWS_SEEDPREP, ST_ROWS, ST_INV_COLS, ST_PACK, ST_COUNT };
+             ST_ROWS_SEEDPREP, ST_ROWS, ST_INV_COLS, ST_PACK, ST_TRANSPOSE_SEED, ST_FWD_COLS_SEED, ST_COUNT };
 const char* const kStageNames[ST_COUNT] = {
     "k_small", "k_zero_u32", "k_transpose_bits(key)", "k_fwd_cols(key)", "k_transpose_bits(key+seed)",
-    "k_fwd_cols(key+seed)", "k_rows(seedprep)", "k_rows", "k_inv_cols", "k_pack"};
+    "k_fwd_cols(key+seed)", "k_rows(seedprep)", "k_rows", "k_inv_cols", "k_pack", "k_transpose_bits(seed)",
+    "k_fwd_cols(seed)"};
+static_assert(ST_COUNT <= 16, "stage tables hold 16 entries");
+
+// A fresh seed still in host memory (pa_run_block): it is copied to dev on the upload stream
+// after the key's copy (event key_up; both share the bus, so running them together would only
+// delay the key) and while the key's first passes run; the seed's passes wait for it.
+struct SeedUpload {
+    const uint32_t* host;
+    uint32_t* dev;
+    size_t bytes;
+    hipStream_t up;
+    hipEvent_t key_up, done;
+};
 
 hipEvent_t pool_event(pa_engine* e) {
     if (!e->event_pool.empty()) {
@@ -424,9 +439,11 @@ hipError_t timed(pa_engine* e, int stage, hipStream_t s, F&& launch) {
     return r;
 }
 
-// The hot path on device buffers.  sspec/ws_dev != null selects the resident seed.
+// The hot path on device buffers.  sspec/ws_dev != null selects the resident seed; su != null
+// brings a fresh seed up from host memory during the key's passes (seed = su->dev).
 int run_device(pa_engine* e, const Geom& g, Workspace* w, const uint32_t* key, const uint32_t* seed,
-               const float2* sspec, const uint32_t* ws_dev, uint32_t* out, hipStream_t s, float* margin) {
+               const float2* sspec, const uint32_t* ws_dev, uint32_t* out, hipStream_t s, float* margin,
+               const SeedUpload* su = nullptr) {
     const pa::Tables t = tables(e, g.L);
     if (g.N1 == 1) {
         PA_HIP(timed(e, ST_SMALL, s, [&] {
@@ -439,7 +456,18 @@ int run_device(pa_engine* e, const Geom& g, Workspace* w, const uint32_t* key, c
     if (!w->counts_zero)
         PA_HIP(timed(e, ST_ZERO, s, [&] { return pa::launch_zero_u32(w->counts, 2 * pa::POP_SLOTS, s); }));
     w->counts_zero = false;  // until this block's pack stage is enqueued
-    if (!sspec) {  // key and seed share each launch of the bit transpose and of pass A
+    if (!sspec && su) {  // key passes first, the seed's after its upload
+        PA_HIP(timed(e, ST_TRANSPOSE_KEY, s, [&] { return pa::launch_transpose(key, w->tbk, key_cnt, g, 0, s); }));
+        PA_HIP(timed(e, ST_FWD_COLS_KEY, s, [&] { return pa::launch_fwd_cols(w->tbk, w->Yk, g, t, s); }));
+        PA_HIP(hipStreamWaitEvent(su->up, su->key_up, 0));
+        PA_HIP(hipMemcpyAsync(su->dev, su->host, su->bytes, hipMemcpyHostToDevice, su->up));
+        PA_HIP(hipEventRecord(su->done, su->up));
+        PA_HIP(hipStreamWaitEvent(s, su->done, 0));
+        PA_HIP(timed(e, ST_TRANSPOSE_SEED, s, [&] { return pa::launch_transpose(su->dev, w->tbs, seed_cnt, g, 1, s); }));
+        PA_HIP(timed(e, ST_FWD_COLS_SEED, s, [&] { return pa::launch_fwd_cols(w->tbs, w->Ys, g, t, s); }));
+        PA_HIP(timed(e, ST_ROWS, s, [&] { return pa::launch_rows(pa::ROWS_DYN, w->Yk, w->Ys, g, t, e->cus, s); }));
+        ws_dev = seed_cnt;
+    } else if (!sspec) {  // key and seed share each launch of the bit transpose and of pass A
         PA_HIP(timed(e, ST_TRANSPOSE_BOTH, s, [&] {
             return pa::launch_transpose(key, w->tbk, key_cnt, g, 0, s, seed, w->tbs);
         }));
@@ -489,6 +517,9 @@ int pa_create(const pa_config* cfg, pa_engine** out) {
     if (e->cfg.max_n_log2 == 0) e->cfg.max_n_log2 = 27;
     PA_HIP(hipSetDevice(e->cfg.device));
     PA_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    PA_HIP(hipStreamCreateWithFlags(&e->upload, hipStreamNonBlocking));
+    PA_HIP(hipEventCreateWithFlags(&e->upload_done, hipEventDisableTiming));
+    PA_HIP(hipEventCreateWithFlags(&e->key_up, hipEventDisableTiming));
     PA_HIP(hipDeviceGetAttribute(&e->cus, hipDeviceAttributeMultiprocessorCount, e->cfg.device));
     int rc = slot_init(&e->batch, false);
     if (rc != PA_OK) return rc;
@@ -540,6 +571,10 @@ int pa_destroy(pa_engine* e) {
     (void)hipFree(e->tw14);
     (void)hipFree(e->cls);
     (void)hipStreamDestroy(e->stream);
+    (void)hipStreamSynchronize(e->upload);
+    (void)hipStreamDestroy(e->upload);
+    (void)hipEventDestroy(e->upload_done);
+    (void)hipEventDestroy(e->key_up);
     delete e;
     return PA_OK;
 }
@@ -646,12 +681,17 @@ static int run_host(pa_engine* e, const uint32_t* key, const uint32_t* seed, pa_
     }
     hipStream_t s = e->stream;
     const uint32_t kw = key_words_for(g);
-    // zero the word after the key so key-rest reads past a NoXOR key stay defined
-    PA_HIP(hipMemsetAsync(w->d_key, 0, (size_t)(g.n / 32 + 2) * 4, s));
+    // zero the words after the key so key-rest reads past a NoXOR key stay defined
+    PA_HIP(hipMemsetAsync(w->d_key + kw, 0, (size_t)(g.n / 32 + 2 - kw) * 4, s));
     PA_HIP(hipMemcpyAsync(w->d_key, key, (size_t)kw * 4, hipMemcpyHostToDevice, s));
-    if (seed) PA_HIP(hipMemcpyAsync(w->d_seed, seed, (size_t)(n / 32) * 4, hipMemcpyHostToDevice, s));
+    // a fresh seed of a multi-pass block goes up while the key's first passes run (the previous
+    // call's kernels, the only other readers of d_seed, finished before it returned)
+    const SeedUpload su{seed, w->d_seed, (size_t)(n / 32) * 4, e->upload, e->key_up, e->upload_done};
+    const bool overlap = seed && g.N1 > 1;
+    if (overlap) PA_HIP(hipEventRecord(e->key_up, s));
+    if (seed && !overlap) PA_HIP(hipMemcpyAsync(w->d_seed, seed, (size_t)(n / 32) * 4, hipMemcpyHostToDevice, s));
     rc = run_device(e, g, w, w->d_key, seed ? w->d_seed : nullptr, sd ? sd->spec : nullptr,
-                    sd ? sd->ws : nullptr, w->d_out, s, nullptr);
+                    sd ? sd->ws : nullptr, w->d_out, s, nullptr, overlap ? &su : nullptr);
     if (rc != PA_OK) return rc;
     PA_HIP(hipMemcpyAsync(out, w->d_out, out_bytes_of(g), hipMemcpyDeviceToHost, s));
     PA_HIP(hipStreamSynchronize(s));
