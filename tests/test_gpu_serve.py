@@ -166,6 +166,40 @@ def test_serve_large_blocks_by_dma(engine, oracle):
         engine.seed_drop(sid)
 
 
+def test_serve_bad_blocks_fail_alone(engine, oracle):
+    """Blocks whose sizes the engine refuses (m not a multiple of 32, m >= n, a non-2^k n without
+    compression, an unknown resident seed) fail with their own status inside a bulk submission;
+    the valid blocks around them complete."""
+    engine.serve_start(slots=2, max_batch=16)
+    try:
+        say("serve bulk with refused blocks")
+        n, m = 1 << 18, 3 << 15
+        good, reqs = [], []
+        for i in range(8):
+            key = oracle.synth_bits(0xE100 + i, n + 1)
+            seed = oracle.synth_bits(0xE200 + i, n)
+            out = np.zeros(m // 8 + 8, np.uint8)
+            reqs.append((key, n, m, out, seed, 0))
+            good.append((out, key, seed))
+        bad = [(oracle.synth_bits(1, n + 1), n, m + 16, np.zeros(m // 8 + 8, np.uint8), oracle.synth_bits(2, n), 0),
+               (oracle.synth_bits(3, n + 1), n, n, np.zeros(n // 8 + 8, np.uint8), oracle.synth_bits(4, n), 0),
+               (oracle.synth_bits(5, 100001), 100000 // 32 * 32, 3200, np.zeros(4096, np.uint8), None, 987654)]
+        order = [0, 1, 100, 2, 3, 101, 4, 5, 102, 6, 7]
+        allreq = [reqs[i] if i < 100 else bad[i - 100] for i in order]
+        tickets = engine.serve_submit_many(allreq)
+        for t, i in zip(tickets, order):
+            if i < 100:
+                engine.serve_wait(t, 60 * 10**9)
+                out, key, seed = good[i]
+                assert np.array_equal(out[: m // 8], _ref(oracle, key, seed, n, m)), i
+            else:
+                with pytest.raises(P.EngineError) as ei:
+                    engine.serve_wait(t, 60 * 10**9)
+                assert ei.value.status < 0, i
+    finally:
+        engine.serve_stop()
+
+
 def test_serve_budget_admission(engine, oracle):
     """A small budget splits the stream into small batches; a block larger than a slot's share of
     the budget is refused with PA_ERR_NOMEM and the others still complete."""
