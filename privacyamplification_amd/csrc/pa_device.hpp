@@ -134,17 +134,6 @@ __host__ __device__ constexpr int seq_stride(int len) { return padi(len) + 1; }
 constexpr int TW14_LOG = 14;
 constexpr int TW14 = 1 << TW14_LOG;
 
-// Hides a pointer's value from the optimiser.  Twiddle tables are read-only, so their loads
-// would otherwise be merged across FFTs and hoisted across barriers into earlier stages, keeping
-// whole stages of twiddles live in registers; laundering the table pointer at the start of each
-// stage pins its loads after the preceding barrier.
-template <typename T>
-__device__ __forceinline__ const T* opaque_ptr(const T* p) {
-    uint64_t v = reinterpret_cast<uint64_t>(p);
-    asm volatile("" : "+s"(v));
-    return reinterpret_cast<const T*>(v);
-}
-
 // exp(-2 pi i y / 2^14), y in [0, 2^14), from a two-level table kept in LDS:
 // twl[0..128) = w^i, twl[128..256) = w^(128 i).  Two LDS reads and one product (about 1.5 ulp).
 constexpr int TWL_SIZE = 256;
@@ -173,8 +162,8 @@ __device__ __forceinline__ void twiddle_powers(float2 (&tw)[R], const float2* tw
     }
 }
 
-// Hides an integer's value from the optimiser (same purpose as opaque_ptr, for per-lane
-// indices: address arithmetic repeated in two phases is otherwise computed once and kept live).
+// Hides an integer's value from the optimiser: per-lane address arithmetic repeated in two phases
+// of a stage is otherwise computed once and kept live across them.
 __device__ __forceinline__ int opaque_int(int x) {
     asm volatile("" : "+v"(x));
     return x;
