@@ -1335,7 +1335,7 @@ struct Server {
 namespace {
 
 // Retires finished batches: outputs from pinned staging to the callers' buffers, then completion.
-bool serve_retire(pa_engine* e, Server* sv) {
+bool serve_retire(Server* sv) {
     bool any = false;
     for (auto& sl : sv->slots) {
         if (!sl.busy) continue;
@@ -1532,7 +1532,7 @@ bool serve_admit(pa_engine* e, Server* sv, ServeSlot& sl) {
 void serve_loop(pa_engine* e, Server* sv) {
     (void)hipSetDevice(e->cfg.device);
     while (true) {
-        bool progressed = serve_retire(e, sv);
+        bool progressed = serve_retire(sv);
         for (auto& sl : sv->slots)
             if (!sl.busy && serve_admit(e, sv, sl)) progressed = true;
         bool busy = false;
