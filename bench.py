@@ -81,7 +81,7 @@ def cpu_baseline(n, m, budget_s=30.0):
     dt = time.perf_counter() - t0
     return {"value": round(n / dt / 1e9, 6), "unit": "Gbit/s", "cores": threads, "kind": "port",
             "sample": f"1 block n=2^{n.bit_length()-1} m=3n/8 dynamic seed, FP32 FFT restatement of the "
-                      f"reference pipeline (oracle/pa_oracle.c, OpenMP), {dt:.1f} s"}
+                      f"reference pipeline (oracle/pa_oracle.c, OpenMP), {dt:.3f} s"}
 
 
 def batched_workload(count=256, master=0xB47C):
