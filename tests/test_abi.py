@@ -1,7 +1,9 @@
 """The C ABI library loads and exports every entry point include/pa_engine.h declares (CPU only:
-no compute calls), and its host-side helpers agree with the oracle."""
+no compute calls), its host-side helpers agree with the oracle, and bench.py keeps its contract."""
 import pathlib
 import re
+import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -98,3 +100,23 @@ def test_server_config_errors_exit_like_the_reference(tmp_path):
     assert _server(tmp_path, "do_compress: maybe\n").returncode == 102     # bad boolean
     assert _server(tmp_path, "factor_exp: 5  # too small\n").returncode == 400 & 0xFF  # 8-bit status
     assert _server(tmp_path, "factor_exp: 27\nverify_ampout: true\nverify_ampout_sha3: 'abc'\n").returncode == 102
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; asked for more it says so and exits non-zero before any GPU work."""
+    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode != 0
+    assert "one GPU" in (r.stdout + r.stderr)
+
+
+def test_bench_algorithmic_bytes_model():
+    """The roofline's byte model: 24n + n/4 + m/4 per dynamic block, 20n + n/8 + m/4 static, and the
+    row pass moves 12n (two spectra in, R' out)."""
+    sys.path.insert(0, str(ROOT))
+    import bench
+    n, m = 1 << 27, 3 << 24
+    assert bench.algorithmic_bytes(n, m, "dynamic") == 24 * n + n / 4 + m / 4
+    assert bench.algorithmic_bytes(n, m, "static") == 20 * n + n / 8 + m / 4
+    assert bench.kernel_bytes(n, m)["k_rows"] == 12 * n
+    assert abs(bench.algorithmic_bytes(n, m, "dynamic") - 3.267e9) < 1e6
