@@ -96,12 +96,16 @@ int pa_run_block_seeded(pa_engine* eng, const uint32_t* key, pa_seed_id seed, ui
  * runs a dynamic seed; seed == NULL uses the resident seed seed_id. */
 int pa_submit(pa_engine* eng, const uint32_t* key, const uint32_t* seed, pa_seed_id seed_id,
               uint32_t n, uint32_t m, uint32_t flags, uint8_t* out, pa_ticket* ticket);
-/* timeout_ns < 0 waits forever. */
+/* timeout_ns < 0 waits forever.  One waiting thread per ticket: a second pa_wait on a ticket
+ * that another thread is waiting on returns PA_ERR_ARG; after success the ticket is gone
+ * (PA_ERR_NOT_FOUND).  A timed-out ticket may be waited on again. */
 int pa_wait(pa_engine* eng, pa_ticket ticket, int64_t timeout_ns);
 
 /* Device-resident form: key, seed and out are device pointers, the work is stream-ordered on
  * `stream` (a hipStream_t, NULL = the engine's stream) and nothing is synchronised.  The engine's
- * per-size workspace is shared, so calls on one engine must not overlap on different streams.
+ * per-size workspace is shared: a call on another stream than the previous call of the same
+ * transform length first waits (on the device) for that call's kernels, so calls on several
+ * streams serialise per size.  Inside a graph capture on `stream` no such wait is recorded.
  * margin_out (device float, may be NULL) receives max |v - rint(v)| over the block's pre-rounding
  * values (atomic max; the caller zeroes it). */
 int pa_block_device(pa_engine* eng, const uint32_t* d_key, const uint32_t* d_seed, uint32_t n,
@@ -188,9 +192,12 @@ int pa_serve_submit(pa_engine* eng, const pa_block* block, pa_ticket* ticket);
  * queued. */
 int pa_serve_submit_many(pa_engine* eng, const pa_block* blocks, uint32_t count, pa_ticket* tickets);
 /* Waits for a block (timeout_ns < 0: forever).  On success *latency_ns (may be NULL) receives the
- * time from pa_serve_submit to the output being written; returns the block's own status. */
+ * time from pa_serve_submit to the output being written; returns the block's own status.  One
+ * waiting thread per ticket (a second concurrent wait returns PA_ERR_ARG). */
 int pa_serve_wait(pa_engine* eng, pa_ticket ticket, int64_t timeout_ns, uint64_t* latency_ns);
-/* Drains every queued and in-flight block, then stops the scheduler. */
+/* Drains every queued and in-flight block, lets the threads already inside pa_serve_wait return,
+ * then stops the scheduler.  No pa_serve_* call may start once pa_serve_stop has begun.
+ * pa_destroy calls it first when a server is running. */
 int pa_serve_stop(pa_engine* eng);
 
 /* Per-stage device timing, the counterpart of the reference's STOPWATCH stage timers
@@ -204,6 +211,28 @@ int pa_stage_times(pa_engine* eng, const char** names, double* ms, uint32_t* cou
 
 /* Bytes of device workspace a block of this shape uses (0 for an unsupported shape). */
 uint64_t pa_workspace_bytes(uint32_t n, uint32_t m, uint32_t flags);
+
+/* The device buffers of one block. */
+enum pa_buffer {
+    PA_BUF_KEY = 0,        /* key words (caller's, or the engine's staging of them) */
+    PA_BUF_SEED,           /* seed words */
+    PA_BUF_OUT,            /* output bytes */
+    PA_BUF_SPEC_KEY,       /* key spectrum, then the product (four-step path) */
+    PA_BUF_SPEC_SEED,      /* seed spectrum */
+    PA_BUF_BITS_KEY,       /* key bit masks of pass A */
+    PA_BUF_BITS_SEED,      /* seed bit masks */
+    PA_BUF_NIB,            /* rounded output bits before packing */
+    PA_BUF_COUNTS,         /* popcount partials */
+    PA_BUF_COUNT
+};
+/* Buffer extents of a block shape, a diagnostic with no counterpart in the reference: touched[b]
+ * is the number of bytes of buffer b that the block's kernels index at most, alloc[b] the bytes
+ * every allocation path gives it (the smallest over the per-size workspace, pa_submit, batch and
+ * serving staging; for key, seed and out also the caller contract of the device entry points:
+ * key_words * 4, n/8 and m/8 or n/16 bytes).  The engine refuses a shape with touched > alloc
+ * (PA_ERR_UNSUPPORTED).  *transform_len receives L.  Runs on the host only. */
+int pa_block_extents(uint32_t n, uint32_t m, uint32_t flags, uint32_t* transform_len,
+                     uint64_t touched[PA_BUF_COUNT], uint64_t alloc[PA_BUF_COUNT]);
 
 /* Synthetic RefBits: xoshiro256** seeded through splitmix64, Bernoulli(p) bits; p == 0.5 takes
  * two words per 64-bit draw (high half first).  Host memory, nwords = ceil(nbits/32). */

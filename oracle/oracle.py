@@ -51,6 +51,11 @@ def lib():
         L.pa_or_amp_out.argtypes = [u32p, u32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_int, u8p, ctypes.c_int]
         L.pa_or_amp_out.restype = ctypes.c_int64
+        L.pa_or_fft32_seed_spectrum.argtypes = [u32p, ctypes.c_uint32, f32p, ctypes.POINTER(ctypes.c_uint32)]
+        L.pa_or_fft32_seed_spectrum.restype = ctypes.c_int
+        L.pa_or_amp_out_fft32_seeded.argtypes = [u32p, f32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                                 ctypes.c_int, ctypes.c_int, ctypes.c_int, u8p]
+        L.pa_or_amp_out_fft32_seeded.restype = ctypes.c_int64
         L.pa_or_fft_values.argtypes = [u32p, ctypes.c_uint32, u32p, ctypes.c_uint32, ctypes.c_uint32,
                                        ctypes.c_int, f64p]
         L.pa_or_fft_values.restype = ctypes.c_int
@@ -152,6 +157,27 @@ def amp_out(key, seed, n, m, do_xor=True, do_compress=True, reverse_endian=True,
     if rc < 0:
         raise ValueError("pa_or_amp_out: bad arguments")
     return out[: (out_bits + 7) // 8] if reverse_endian else out[: ((out_bits + 31) // 32) * 4]
+
+
+def fft32_seed_spectrum(seed, n):
+    """The static-seed FP32 restatement's per-seed half: (S, ws) for amp_out_fft32_seeded."""
+    S = np.zeros(n + 2, dtype=np.float32)
+    ws = ctypes.c_uint32()
+    if lib().pa_or_fft32_seed_spectrum(np.ascontiguousarray(seed, dtype=np.uint32), n, S, ctypes.byref(ws)) != 0:
+        raise MemoryError("pa_or_fft32_seed_spectrum")
+    return S, ws.value
+
+
+def amp_out_fft32_seeded(key, spectrum, n, m, do_xor=True, do_compress=True, reverse_endian=True):
+    """One block of the FP32 restatement with a precomputed seed spectrum (static seed)."""
+    S, ws = spectrum
+    out_bits = m if do_compress else n // 2
+    out = np.zeros(((out_bits + 31) // 32) * 4, dtype=np.uint8)
+    rc = lib().pa_or_amp_out_fft32_seeded(np.ascontiguousarray(key, dtype=np.uint32), S, ws, n, m, int(do_xor),
+                                          int(do_compress), int(reverse_endian), out)
+    if rc < 0:
+        raise ValueError("pa_or_amp_out_fft32_seeded: bad arguments")
+    return out[: (out_bits + 7) // 8] if reverse_endian else out
 
 
 def set_ntt_big(on):

@@ -524,31 +524,54 @@ static int conv_ntt(const uint32_t* ks, uint32_t kbits, const uint32_t* seed, ui
 DEFINE_FFT(float, f32)
 DEFINE_FFT(double, f64)
 
+/* The seed half of the reference's FP32 pipeline (binInt2float and cufftExecR2C of the seed,
+ * PrivacyAmplification.cu:1819-1830, 1903; DC zeroed as setFirstElementToZero does at :1946):
+ * S = n/2+1 complex (n+2 floats), *ws = the seed's popcount.  The reference computes it once
+ * and reuses it while a seed is reused (reuse_seed_amount != 0, :1781-1791, 1898-1910). */
+int pa_or_fft32_seed_spectrum(const uint32_t* seed, uint32_t n, float* S, uint32_t* ws) {
+    const uint32_t R = 11;
+    float* seed_f = (float*)calloc((size_t)n + 2, 4);
+    float* tmp = (float*)malloc(sizeof(float) * ((size_t)n + 2));
+    if (!seed_f || !tmp) { free(seed_f); free(tmp); return -1; }
+    *ws = pa_or_bin_int2float(seed, n, R, seed_f);
+    r2c_f32(seed_f, S, tmp, n);
+    S[0] = S[1] = 0.0f;
+    free(seed_f); free(tmp);
+    return 0;
+}
+
 /* The reference's FP32 pipeline up to the float values ToBinaryArray consumes.
- * Returns v (n floats, cuFFT-style unnormalized C2R output), corr and norm. */
-static int fft32_pipeline(const uint32_t* ks, uint32_t kbits, const uint32_t* seed, uint32_t n,
-                          float* v, float* corr_out, float* norm_out) {
+ * Returns v (n floats, cuFFT-style unnormalized C2R output), corr and norm.  S_static != NULL:
+ * the seed's spectrum from pa_or_fft32_seed_spectrum (static seed), ws_static its popcount. */
+static int fft32_pipeline_s(const uint32_t* ks, uint32_t kbits, const uint32_t* seed, uint32_t n,
+                            const float* S_static, uint32_t ws_static, float* v, float* corr_out,
+                            float* norm_out) {
     const uint32_t R = 11, Q = 5; /* reduction_exp, pre_mul_reduction_exp (config.yaml:42,47) */
     float* key_f = (float*)calloc((size_t)n + 2, 4);
-    float* seed_f = (float*)calloc((size_t)n + 2, 4);
     float* X = (float*)malloc(sizeof(float) * ((size_t)n + 2));
-    float* S = (float*)malloc(sizeof(float) * ((size_t)n + 2));
+    float* S = S_static ? NULL : (float*)malloc(sizeof(float) * ((size_t)n + 2));
     float* tmp = (float*)malloc(sizeof(float) * ((size_t)n + 2));
-    if (!key_f || !seed_f || !X || !S || !tmp) {
-        free(key_f); free(seed_f); free(X); free(S); free(tmp); return -1;
+    if (!key_f || !X || (!S_static && !S) || !tmp) {
+        free(key_f); free(X); free(S); free(tmp); return -1;
+    }
+    uint32_t ws = ws_static;
+    if (!S_static && pa_or_fft32_seed_spectrum(seed, n, S, &ws) != 0) {
+        free(key_f); free(X); free(S); free(tmp); return -1;
     }
     uint32_t wk = pa_or_bin_int2float(ks, kbits, R, key_f);
-    uint32_t ws = pa_or_bin_int2float(seed, n, R, seed_f);
     *corr_out = pa_or_correction(wk, ws, n);
     r2c_f32(key_f, X, tmp, n);
-    r2c_f32(seed_f, S, tmp, n);
-    X[0] = X[1] = 0.0f; S[0] = S[1] = 0.0f;                       /* setFirstElementToZero */
-    pa_or_element_wise_product(X, S, n / 2 + 1, (float)(1u << Q)); /* ElementWiseProduct */
+    X[0] = X[1] = 0.0f;                                           /* setFirstElementToZero */
+    pa_or_element_wise_product(X, S_static ? S_static : S, n / 2 + 1, (float)(1u << Q)); /* ElementWiseProduct */
     c2r_f32(X, v, tmp, n);
     const float total_reduction = (float)((1u << R) * (1u << Q));
     *norm_out = ((float)n) / total_reduction / total_reduction;    /* PrivacyAmplification.cu:1729-1730 */
-    free(key_f); free(seed_f); free(X); free(S); free(tmp);
+    free(key_f); free(X); free(S); free(tmp);
     return 0;
+}
+static int fft32_pipeline(const uint32_t* ks, uint32_t kbits, const uint32_t* seed, uint32_t n,
+                          float* v, float* corr_out, float* norm_out) {
+    return fft32_pipeline_s(ks, kbits, seed, n, NULL, 0, v, corr_out, norm_out);
 }
 
 /* The same convolution in FP64 with 0/1 inputs; v[p] = conv + corr with the DC removed, which is
@@ -628,8 +651,9 @@ int pa_or_conv_bits(const uint32_t* ks, uint32_t kbits, const uint32_t* seed, ui
 }
 
 /* ---------------------------------------------------------------- one block ---- */
-int64_t pa_or_amp_out(const uint32_t* key, const uint32_t* seed, uint32_t n, uint32_t m,
-                      int do_xor, int do_compress, int reverse_endian, uint8_t* out, int method) {
+static int64_t amp_out_impl(const uint32_t* key, const uint32_t* seed, uint32_t n, uint32_t m,
+                            int do_xor, int do_compress, int reverse_endian, uint8_t* out, int method,
+                            const float* S_static, uint32_t ws_static) {
     if (n < 64 || (n % 32) || m == 0 || (m % 32) || m >= n) return -1;
     uint32_t h = n - m, kbits = h + 1;
     uint32_t out_bits = do_compress ? m : n / 2;
@@ -655,7 +679,7 @@ int64_t pa_or_amp_out(const uint32_t* key, const uint32_t* seed, uint32_t n, uin
         /* ToBinaryArray consumes the float values directly. */
         float* v = (float*)malloc(sizeof(float) * ((size_t)n + 2));
         float corr, norm;
-        rc = v ? fft32_pipeline(ks, kbits, seed, n, v, &corr, &norm) : -1;
+        rc = v ? fft32_pipeline_s(ks, kbits, seed, n, S_static, ws_static, v, &corr, &norm) : -1;
         if (rc == 0) {
             uint32_t nwo = (out_bits + 31) / 32;
             uint32_t* words = (uint32_t*)calloc(nwo, 4);
@@ -689,4 +713,15 @@ int64_t pa_or_amp_out(const uint32_t* key, const uint32_t* seed, uint32_t n, uin
     }
     free(ks); free(kr); free(conv);
     return rc == 0 ? (int64_t)out_bits : -1;
+}
+
+int64_t pa_or_amp_out(const uint32_t* key, const uint32_t* seed, uint32_t n, uint32_t m,
+                      int do_xor, int do_compress, int reverse_endian, uint8_t* out, int method) {
+    return amp_out_impl(key, seed, n, m, do_xor, do_compress, reverse_endian, out, method, NULL, 0);
+}
+
+int64_t pa_or_amp_out_fft32_seeded(const uint32_t* key, const float* S, uint32_t ws, uint32_t n, uint32_t m,
+                                   int do_xor, int do_compress, int reverse_endian, uint8_t* out) {
+    if (!S) return -1;
+    return amp_out_impl(key, NULL, n, m, do_xor, do_compress, reverse_endian, out, PA_OR_FFT32, S, ws);
 }

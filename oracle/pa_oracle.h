@@ -68,6 +68,14 @@ int pa_or_conv_bits(const uint32_t* key_start, uint32_t kbits, const uint32_t* s
 int64_t pa_or_amp_out(const uint32_t* key, const uint32_t* seed, uint32_t n, uint32_t m,
                       int do_xor, int do_compress, int reverse_endian, uint8_t* out, int method);
 
+/* Static-seed form of the FP32 restatement: the seed's half spectrum S (n+2 floats, DC zeroed)
+ * and popcount ws are computed once by pa_or_fft32_seed_spectrum, and each block then runs only
+ * the key's transform, the product and the inverse (reuse_seed_amount != 0,
+ * PrivacyAmplification.cu:1781-1791, 1898-1910).  Same output as pa_or_amp_out(..., PA_OR_FFT32). */
+int pa_or_fft32_seed_spectrum(const uint32_t* seed, uint32_t n, float* S, uint32_t* ws);
+int64_t pa_or_amp_out_fft32_seeded(const uint32_t* key, const float* S, uint32_t ws, uint32_t n, uint32_t m,
+                                   int do_xor, int do_compress, int reverse_endian, uint8_t* out);
+
 /* Pre-rounding values of the FP32 (or FP64) FFT restatement: v[p] = conv/norm + corr for p < nout,
  * i.e. the numbers ToBinaryArray rounds.  Used to measure precision margins. */
 int pa_or_fft_values(const uint32_t* key_start, uint32_t kbits, const uint32_t* seed, uint32_t n,

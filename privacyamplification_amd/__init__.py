@@ -6,6 +6,7 @@ hand-written HIP kernels behind the C ABI declared in include/pa_engine.h.
 """
 from .engine import (  # noqa: F401
     BATCH_GRAPH,
+    BUFFERS,
     BATCH_TUNE,
     COMPRESS,
     DEFAULT_FLAGS,
@@ -14,6 +15,7 @@ from .engine import (  # noqa: F401
     EngineError,
     PaBlock,
     PaServeConfig,
+    block_extents,
     build,
     library_path,
     load_library,

@@ -100,6 +100,70 @@ int make_geom(uint32_t n, uint32_t m, uint32_t flags, uint32_t reverse_endian, u
 
 uint32_t key_words_for(const Geom& g) { return g.do_xor ? g.n / 32 + 1 : g.key_hw + 1; }
 
+// ------------------------------------------------------------------ buffer extents ----
+// Every device buffer of one block: the bytes its kernels index at most (`touched`, from the
+// kernels' own index arithmetic) and the bytes every allocation path gives it (`alloc`, the
+// smallest over the workspace, pa_submit, batch, serving and split paths and the caller contract
+// of the device entry points).  All allocation sites take their sizes from the stage_* functions
+// below, and prepare() refuses a shape whose touched bytes exceed its allocation.
+size_t out_bytes_of(const Geom& g) { return g.out_bits / 8; }
+// engine staging of one key: n/32 + 2 words (the kernels read n/32 + 1 at most; one zero word
+// more keeps a NoXOR key's staging defined past its h/32 + 1 words)
+uint64_t stage_key_bytes(const Geom& g) { return (uint64_t)(g.n / 32 + 2) * 4; }
+uint64_t stage_seed_bytes(const Geom& g) { return (uint64_t)(g.n / 32) * 4; }
+// engine staging of one output: the output plus 16 bytes
+uint64_t stage_out_bytes(const Geom& g) { return out_bytes_of(g) + 16; }
+// a length-L workspace's staging, shared by every block of that length (n <= L; the output, m/8
+// bytes with compression and n/16 without, m < n, plus 16 fits in L/8 + 16)
+uint64_t ws_key_bytes(uint32_t L) { return (uint64_t)(L / 32 + 2) * 4; }
+uint64_t ws_out_bytes(uint32_t L) { return (uint64_t)L / 8 + 16; }
+
+struct Extents {
+    uint64_t touched[PA_BUF_COUNT] = {};
+    uint64_t alloc[PA_BUF_COUNT] = {};
+    bool ok() const {
+        for (int i = 0; i < PA_BUF_COUNT; ++i)
+            if (touched[i] > alloc[i]) return false;
+        return true;
+    }
+};
+
+Extents block_extents(const Geom& g) {
+    Extents x;
+    const uint64_t N = g.N;
+    // key: k_transpose_bits / k_small read words [0, h/32] through key_word(); with the key-rest
+    // XOR, k_pack / k_small read key[h/32 + w + 1] for 32 w < m, i.e. up to word n/32
+    x.touched[PA_BUF_KEY] = (uint64_t)(g.do_xor ? g.n / 32 + 1 : g.key_hw + 1) * 4;
+    x.touched[PA_BUF_SEED] = (uint64_t)g.seed_words * 4;  // seed_word(): words [0, n/32)
+    // k_pack (four-step) and k_small write output words [0, out_bits / 32)
+    x.touched[PA_BUF_OUT] = (uint64_t)g.out_bits / 8;
+    x.alloc[PA_BUF_KEY] = std::min(stage_key_bytes(g), ws_key_bytes(g.L));
+    x.alloc[PA_BUF_SEED] = std::min(stage_seed_bytes(g), ws_key_bytes(g.L));
+    // the device entry points' caller contract: m/8 bytes (compress) or n/16; staging is larger
+    x.alloc[PA_BUF_OUT] = std::min({(uint64_t)out_bytes_of(g), stage_out_bytes(g), ws_out_bytes(g.L)});
+    x.alloc[PA_BUF_COUNTS] = 2 * pa::POP_SLOTS * 4;
+    if (g.N1 == 1) return x;  // the one-workgroup path keeps everything else on chip
+    x.touched[PA_BUF_COUNTS] = 2 * pa::POP_SLOTS * 4;
+    // k_transpose_bits writes tb[u N2 + t2] for u < M, t2 < N2
+    x.touched[PA_BUF_BITS_KEY] = x.touched[PA_BUF_BITS_SEED] = (uint64_t)g.M * g.N2 * 4;
+    // pass A writes Y[k1 N2 + t2], k1 < N1; the seed-spectrum pass adds the Nyquist value S[N]
+    x.touched[PA_BUF_SPEC_KEY] = N * 8;
+    x.touched[PA_BUF_SPEC_SEED] = (N + 1) * 8;
+    // pass A' writes crumbs [0, N) (one column per workgroup) or nibbles [0, N/2); k_pack reads
+    // 4 bytes at (column or pair) * N1 + row for rows row < N1, row a multiple of 4
+    {
+        const uint64_t cols = g.W == 1 ? g.N2 : g.N2 / 2;
+        const uint64_t last_row = g.N1 >= 4 ? g.N1 - 4 : 0;
+        const uint64_t wr = cols * g.N1, rd = (cols - 1) * g.N1 + last_row + 4;
+        x.touched[PA_BUF_NIB] = std::max(wr, rd);
+    }
+    x.alloc[PA_BUF_SPEC_KEY] = N * 8;
+    x.alloc[PA_BUF_SPEC_SEED] = (N + 1) * 8;
+    x.alloc[PA_BUF_BITS_KEY] = x.alloc[PA_BUF_BITS_SEED] = (uint64_t)(N / g.D) * 4;
+    x.alloc[PA_BUF_NIB] = pa::nib_bytes(g.N, g.W);
+    return x;
+}
+
 // Per transform length: w_L^x = lo[x mod 2^14] * hi[x >> 14].
 struct Twiddles {
     float2* lo = nullptr;      // 2^14 entries
@@ -120,6 +184,11 @@ struct Workspace {
     uint32_t* d_seed = nullptr;
     uint32_t* d_out = nullptr;
     uint64_t bytes = 0;
+    // The last stream that used this workspace and an event recorded after its last kernel: a
+    // block on another stream waits for it, so calls on two streams serialise instead of
+    // overwriting each other's spectra.
+    hipStream_t last_stream = nullptr;
+    hipEvent_t last_done = nullptr;
 };
 
 struct Seed {
@@ -137,6 +206,7 @@ struct Ticket {
     uint32_t* dev_key = nullptr;
     uint32_t* dev_seed = nullptr;
     size_t out_bytes = 0;
+    bool waiting = false;  // a pa_wait call is polling this ticket's event
 };
 
 // A batch in flight: its device arena (block table, scratch, staging), the pinned host copy of its
@@ -318,7 +388,7 @@ uint64_t workspace_bytes(const Geom& g) {
         b += N * sizeof(float2) + (N + 1) * sizeof(float2);
         b += 2 * (N / g.D) * 4 + pa::nib_bytes(g.N, g.W) + 192;
     }
-    b += 256 + (uint64_t)(g.L / 32 + 2) * 4 * 2 + (g.L / 8 + 16);  // key/seed and output staging
+    b += 256 + ws_key_bytes(g.L) * 2 + ws_out_bytes(g.L);  // key/seed and output staging
     return b;
 }
 
@@ -356,8 +426,7 @@ int get_workspace(pa_engine* e, const Geom& g, Workspace** out) {
     if (rc != PA_OK) return rc;
     auto w = std::make_unique<Workspace>();
     w->L = g.L;
-    // out: m/8 bytes with compression, n/16 without; any m < n <= L fits in L/8
-    const size_t key_bytes = (size_t)(g.L / 32 + 2) * 4, out_bytes = (size_t)(g.L / 8 + 16);
+    const size_t key_bytes = ws_key_bytes(g.L), out_bytes = ws_out_bytes(g.L);
     struct A { void** p; size_t b; };
     std::vector<A> allocs = {
         {(void**)&w->counts, 2 * pa::POP_SLOTS * sizeof(uint32_t)},
@@ -381,6 +450,7 @@ int get_workspace(pa_engine* e, const Geom& g, Workspace** out) {
         }
         w->bytes += a.b;
     }
+    PA_HIP(hipEventCreateWithFlags(&w->last_done, hipEventDisableTiming));
     *out = w.get();
     e->ws.emplace(g.L, std::move(w));
     return PA_OK;
@@ -441,9 +511,31 @@ hipError_t timed(pa_engine* e, int stage, hipStream_t s, F&& launch) {
 
 // The hot path on device buffers.  sspec/ws_dev != null selects the resident seed; su != null
 // brings a fresh seed up from host memory during the key's passes (seed = su->dev).
+int run_device_body(pa_engine* e, const Geom& g, Workspace* w, const uint32_t* key, const uint32_t* seed,
+                    const float2* sspec, const uint32_t* ws_dev, uint32_t* out, hipStream_t s, float* margin,
+                    const SeedUpload* su);
+
+// Orders the workspace's uses across streams (not while s is being captured into a graph: a
+// capture only records work, and the caller owns the ordering of its replays).
 int run_device(pa_engine* e, const Geom& g, Workspace* w, const uint32_t* key, const uint32_t* seed,
                const float2* sspec, const uint32_t* ws_dev, uint32_t* out, hipStream_t s, float* margin,
                const SeedUpload* su = nullptr) {
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    PA_HIP(hipStreamIsCapturing(s, &cap));
+    const bool guard = cap == hipStreamCaptureStatusNone;
+    if (guard && w->last_stream && w->last_stream != s) PA_HIP(hipStreamWaitEvent(s, w->last_done, 0));
+    const int rc = run_device_body(e, g, w, key, seed, sspec, ws_dev, out, s, margin, su);
+    if (rc != PA_OK) return rc;
+    if (guard) {
+        PA_HIP(hipEventRecord(w->last_done, s));
+        w->last_stream = s;
+    }
+    return PA_OK;
+}
+
+int run_device_body(pa_engine* e, const Geom& g, Workspace* w, const uint32_t* key, const uint32_t* seed,
+                    const float2* sspec, const uint32_t* ws_dev, uint32_t* out, hipStream_t s, float* margin,
+                    const SeedUpload* su) {
     const pa::Tables t = tables(e, g.L);
     if (g.N1 == 1) {
         PA_HIP(timed(e, ST_SMALL, s, [&] {
@@ -489,10 +581,9 @@ int run_device(pa_engine* e, const Geom& g, Workspace* w, const uint32_t* key, c
 int prepare(pa_engine* e, uint32_t n, uint32_t m, uint32_t flags, Geom* g, Workspace** w) {
     int rc = make_geom(n, m, flags, e->cfg.out_reverse_endian, e->cfg.max_n_log2, g);
     if (rc != PA_OK) return rc;
+    if (!block_extents(*g).ok()) return PA_ERR_UNSUPPORTED;  // a kernel would index past a buffer
     return get_workspace(e, *g, w);
 }
-
-size_t out_bytes_of(const Geom& g) { return g.out_bits / 8; }
 
 }  // namespace
 
@@ -538,6 +629,10 @@ int pa_create(const pa_config* cfg, pa_engine** out) {
 int pa_destroy(pa_engine* e) {
     if (!e) return PA_ERR_ARG;
     (void)hipSetDevice(e->cfg.device);
+    // The serving scheduler launches kernels that read resident seeds, workspaces and twiddle
+    // tables: drain its queued and in-flight blocks and stop it before anything is freed.
+    if (e->server) pa_serve_stop(e);
+    std::unique_lock<std::mutex> lk(e->mu);
     (void)hipStreamSynchronize(e->stream);
     for (auto& kv : e->tickets) {
         Ticket& t = kv.second;
@@ -554,6 +649,7 @@ int pa_destroy(pa_engine* e) {
         Workspace* w = kv.second.get();
         void* ps[] = {w->Yk, w->Ys, w->tbk, w->tbs, w->nib, w->counts, w->d_key, w->d_seed, w->d_out};
         for (void* p : ps) (void)hipFree(p);
+        if (w->last_done) (void)hipEventDestroy(w->last_done);
     }
     for (auto& kv : e->tw) {
         (void)hipFree(kv.second.lo);
@@ -564,7 +660,6 @@ int pa_destroy(pa_engine* e) {
         (void)hipEventDestroy(p.b);
     }
     for (hipEvent_t ev : e->event_pool) (void)hipEventDestroy(ev);
-    if (e->server) pa_serve_stop(e);
     for (auto& kv : e->pinned) (void)hipHostFree(reinterpret_cast<void*>(kv.first));
     slot_release(e, &e->batch);
     if (e->split) (void)hipFree(e->split);
@@ -575,6 +670,7 @@ int pa_destroy(pa_engine* e) {
     (void)hipStreamDestroy(e->upload);
     (void)hipEventDestroy(e->upload_done);
     (void)hipEventDestroy(e->key_up);
+    lk.unlock();  // the mutex is a member of *e
     delete e;
     return PA_OK;
 }
@@ -583,6 +679,20 @@ uint64_t pa_workspace_bytes(uint32_t n, uint32_t m, uint32_t flags) {
     Geom g;
     if (make_geom(n, m, flags, 1, 31, &g) != PA_OK) return 0;
     return workspace_bytes(g);
+}
+
+int pa_block_extents(uint32_t n, uint32_t m, uint32_t flags, uint32_t* transform_len,
+                     uint64_t touched[PA_BUF_COUNT], uint64_t alloc[PA_BUF_COUNT]) {
+    Geom g;
+    const int rc = make_geom(n, m, flags, 1, pa::MAX_L_LOG, &g);
+    if (rc != PA_OK) return rc;
+    const Extents x = block_extents(g);
+    if (transform_len) *transform_len = g.L;
+    for (int i = 0; i < PA_BUF_COUNT; ++i) {
+        if (touched) touched[i] = x.touched[i];
+        if (alloc) alloc[i] = x.alloc[i];
+    }
+    return PA_OK;
 }
 
 int pa_seed_put(pa_engine* e, const uint32_t* seed, uint32_t n, uint32_t m, int32_t reuse, pa_seed_id* id) {
@@ -727,13 +837,13 @@ int pa_submit(pa_engine* e, const uint32_t* key, const uint32_t* seed, pa_seed_i
     Ticket t;
     t.host_out = out;
     t.out_bytes = out_bytes_of(g);
-    const size_t kbytes = (size_t)(g.n / 32 + 2) * 4;
+    const size_t kbytes = stage_key_bytes(g);
     PA_HIP(hipMalloc(&t.dev_key, kbytes));
-    PA_HIP(hipMalloc(&t.dev_out, t.out_bytes + 16));
-    if (seed) PA_HIP(hipMalloc(&t.dev_seed, (size_t)(n / 32) * 4));
+    PA_HIP(hipMalloc(&t.dev_out, stage_out_bytes(g)));
+    if (seed) PA_HIP(hipMalloc(&t.dev_seed, stage_seed_bytes(g)));
     PA_HIP(hipMemsetAsync(t.dev_key, 0, kbytes, s));
     PA_HIP(hipMemcpyAsync(t.dev_key, key, (size_t)key_words_for(g) * 4, hipMemcpyHostToDevice, s));
-    if (seed) PA_HIP(hipMemcpyAsync(t.dev_seed, seed, (size_t)(n / 32) * 4, hipMemcpyHostToDevice, s));
+    if (seed) PA_HIP(hipMemcpyAsync(t.dev_seed, seed, stage_seed_bytes(g), hipMemcpyHostToDevice, s));
     rc = run_device(e, g, w, t.dev_key, t.dev_seed, sd ? sd->spec : nullptr, sd ? sd->ws : nullptr,
                     t.dev_out, s, nullptr);
     if (rc != PA_OK) return rc;
@@ -752,24 +862,33 @@ int pa_wait(pa_engine* e, pa_ticket ticket, int64_t timeout_ns) {
         std::lock_guard<std::mutex> lk(e->mu);
         auto it = e->tickets.find(ticket);
         if (it == e->tickets.end()) return PA_ERR_NOT_FOUND;
+        if (it->second.waiting) return PA_ERR_ARG;  // one waiter per ticket: the winner destroys the event
+        it->second.waiting = true;
         ev = it->second.done;
     }
+    auto give_up = [&](int st) {
+        std::lock_guard<std::mutex> lk(e->mu);
+        auto it = e->tickets.find(ticket);
+        if (it != e->tickets.end()) it->second.waiting = false;
+        return st;
+    };
     const auto t0 = std::chrono::steady_clock::now();
     while (true) {
         hipError_t q = hipEventQuery(ev);
         if (q == hipSuccess) break;
         if (q != hipErrorNotReady) {
             set_error(q, "hipEventQuery", __FILE__, __LINE__);
-            return PA_ERR_DEVICE;
+            return give_up(PA_ERR_DEVICE);
         }
         if (timeout_ns >= 0) {
             const auto dt = std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - t0);
-            if (dt.count() > timeout_ns) return PA_ERR_TIMEOUT;
+            if (dt.count() > timeout_ns) return give_up(PA_ERR_TIMEOUT);
         }
         std::this_thread::yield();
     }
     std::lock_guard<std::mutex> lk(e->mu);
     auto it = e->tickets.find(ticket);
+    if (it == e->tickets.end()) return PA_ERR_NOT_FOUND;  // another waiter retired it meanwhile
     Ticket t = it->second;
     e->tickets.erase(it);
     (void)hipEventDestroy(t.done);
@@ -1134,14 +1253,14 @@ int pa_run_batch(pa_engine* e, const pa_block* blocks, uint32_t count, uint32_t 
     uint64_t off = p.table_bytes + p.scratch_bytes, key_lo = off;
     for (uint32_t i = 0; i < count; ++i) {
         koff[i] = off;
-        off += align256((uint64_t)(p.items[i].g.n / 32 + 2) * 4);
+        off += align256(stage_key_bytes(p.items[i].g));
     }
     const uint64_t key_hi = off;
     for (uint32_t i = 0; i < count; ++i) {
         soff[i] = off;
-        if (p.items[i].dyn) off += align256((uint64_t)(p.items[i].g.n / 32) * 4);
+        if (p.items[i].dyn) off += align256(stage_seed_bytes(p.items[i].g));
         ooff[i] = off;
-        off += align256(out_bytes_of(p.items[i].g) + 16);
+        off += align256(stage_out_bytes(p.items[i].g));
     }
     rc = ensure_arena(e, e->batch, off);
     if (rc != PA_OK) return rc;
@@ -1155,7 +1274,7 @@ int pa_run_batch(pa_engine* e, const pa_block* blocks, uint32_t count, uint32_t 
         PA_HIP(hipMemcpyAsync(arena + koff[i], blocks[i].key, (size_t)key_words_for(g) * 4, hipMemcpyHostToDevice, s));
         if (p.items[i].dyn) {
             dev[i].seed = reinterpret_cast<const uint32_t*>(arena + soff[i]);
-            PA_HIP(hipMemcpyAsync(arena + soff[i], blocks[i].seed, (size_t)(g.n / 32) * 4, hipMemcpyHostToDevice, s));
+            PA_HIP(hipMemcpyAsync(arena + soff[i], blocks[i].seed, stage_seed_bytes(g), hipMemcpyHostToDevice, s));
         }
         dev[i].out = arena + ooff[i];
     }
@@ -1299,6 +1418,7 @@ struct ServeReq {
     pa_block b{};                  // host pointers
     bool stage_out = false;        // output through the slot's host staging (caller's not pinned)
     int status = 1;                // 1 while pending, then the block's final status
+    bool waited = false;           // a pa_serve_wait call holds this request
     Clock::time_point t_submit, t_done;
     uint64_t dkey = 0, dseed = 0, dout = 0;  // arena offsets
     uint64_t hout = 0;                       // host staging offset of a staged output
@@ -1328,6 +1448,7 @@ struct Server {
     std::vector<ServeSlot> slots;
     uint64_t in_use = 0;
     bool stop = false;
+    int waiters = 0;  // threads inside pa_serve_wait (pa_serve_stop lets them leave first)
     pa_ticket next = 1;
     int init_status = PA_OK;
 };
@@ -1374,9 +1495,8 @@ void serve_fail(Server* sv, ServeReq* r, int st) {
 
 // Device bytes a request holds while in flight: its scratch and its staging.
 uint64_t serve_req_bytes(const Geom& g, bool dyn) {
-    return batch_scratch_bytes(g, dyn) + align256((uint64_t)(g.n / 32 + 2) * 4) +
-           (dyn ? align256((uint64_t)(g.n / 32) * 4) : 0) + align256(out_bytes_of(g) + 16) +
-           align256(sizeof(pa::BlockArgs));
+    return batch_scratch_bytes(g, dyn) + align256(stage_key_bytes(g)) + (dyn ? align256(stage_seed_bytes(g)) : 0) +
+           align256(stage_out_bytes(g)) + align256(sizeof(pa::BlockArgs));
 }
 
 // Forms one batch from the queue head into a free slot and launches it.  Returns false when
@@ -1437,15 +1557,15 @@ bool serve_admit(pa_engine* e, Server* sv, ServeSlot& sl) {
         const Geom& g = p.items[k].g;
         r->key_bytes = (size_t)key_words_for(g) * 4;  // the kernels read no key word beyond these
         r->dkey = off;
-        off += align256((uint64_t)(g.n / 32 + 2) * 4);
-        r->seed_bytes = p.items[k].dyn ? (size_t)(g.n / 32) * 4 : 0;
+        off += align256(stage_key_bytes(g));
+        r->seed_bytes = p.items[k].dyn ? stage_seed_bytes(g) : 0;
         r->dseed = off;
         off += align256(r->seed_bytes);
     }
     for (size_t k = 0; k < ok.size(); ++k) {
         ok[k]->out_bytes = out_bytes_of(p.items[k].g);
         ok[k]->dout = off;
-        off += align256(ok[k]->out_bytes + 16);
+        off += align256(stage_out_bytes(p.items[k].g));
     }
     rc = ensure_arena(e, sl.bs, off, slot_limit);
     if (rc != PA_OK) return fail_all(rc);
@@ -1650,15 +1770,26 @@ int pa_serve_wait(pa_engine* e, pa_ticket ticket, int64_t timeout_ns, uint64_t* 
     auto it = sv->reqs.find(ticket);
     if (it == sv->reqs.end()) return PA_ERR_NOT_FOUND;
     ServeReq* r = it->second;
+    if (r->waited) return PA_ERR_ARG;  // one waiter per ticket: the winner frees the request
+    r->waited = true;
+    ++sv->waiters;
+    auto leave = [&](int st) {
+        --sv->waiters;
+        sv->cv_done.notify_all();
+        return st;
+    };
     auto ready = [&] { return r->status != 1; };
     if (timeout_ns < 0) sv->cv_done.wait(lk, ready);
-    else if (!sv->cv_done.wait_for(lk, std::chrono::nanoseconds(timeout_ns), ready)) return PA_ERR_TIMEOUT;
+    else if (!sv->cv_done.wait_for(lk, std::chrono::nanoseconds(timeout_ns), ready)) {
+        r->waited = false;
+        return leave(PA_ERR_TIMEOUT);
+    }
     const int st = r->status;
     if (latency_ns)
         *latency_ns = (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(r->t_done - r->t_submit).count();
     sv->reqs.erase(it);
     delete r;
-    return st;
+    return leave(st);
 }
 
 int pa_serve_stop(pa_engine* e) {
@@ -1670,6 +1801,12 @@ int pa_serve_stop(pa_engine* e) {
     }
     sv->cv_work.notify_all();
     if (sv->th.joinable()) sv->th.join();
+    {
+        // every request has its final status now: let threads already inside pa_serve_wait leave
+        std::unique_lock<std::mutex> lk(sv->mu);
+        sv->cv_done.notify_all();
+        sv->cv_done.wait(lk, [&] { return sv->waiters == 0; });
+    }
     for (auto& sl : sv->slots) {
         slot_release(e, &sl.bs);
         if (sl.done) (void)hipEventDestroy(sl.done);

@@ -12,7 +12,7 @@
 // with its seed reuse rule (a seed with reuse k > 0 serves k blocks, 0 one block, -1 every later one; mainloop
 // :1779-1791, 2099-2101), its file inputs when a server is off (keyfile_path, toeplitz_seed_path),
 // ampout.bin for the first store_first_ampouts_in_file outputs, verify_ampout (SHA3-256 against an
-// expected digest) and the Blocktime line.  Exit codes as the reference: 101 verification failed,
+// expected digest, on a pool of verify_ampout_threads workers) and the Blocktime line.  Exit codes as the reference: 101 verification failed,
 // 102 config, 103/104 seed file missing/short, 105/106 key file missing/short, 400 bad factor_exp.
 //
 // Blocks go through the engine's serving scheduler (pa_serve_*) from pinned buffers
@@ -340,9 +340,15 @@ int main(int argc, char** argv) {
     Queue<Done> done_q;
     int fatal = 0;
     std::mutex fatal_mu;
+    // A fatal error anywhere closes the input queues, so the block loop's waits on them return
+    // and the process ends with the error's code instead of waiting for more input.
     auto set_fatal = [&](int code) {
-        std::lock_guard<std::mutex> lk(fatal_mu);
-        if (!fatal) fatal = code;
+        {
+            std::lock_guard<std::mutex> lk(fatal_mu);
+            if (!fatal) fatal = code;
+        }
+        key_q.close();
+        seed_q.close();
     };
 
     // ---- inputs
@@ -360,6 +366,7 @@ int main(int argc, char** argv) {
         if (!read_file(cfg.toeplitz_seed_path, file_seed.data(), nw * 4ull, 103, 104, code)) return code;
     }
     std::thread key_rx([&] {
+        struct Closer { Queue<KeyItem>& q; ~Closer() { q.close(); } } closer{key_q};  // no more keys
         while (true) {
             int b;
             if (!free_keys.pop(b)) return;
@@ -393,6 +400,7 @@ int main(int argc, char** argv) {
         }
     });
     std::thread seed_rx([&] {
+        struct Closer { Queue<SeedItem>& q; ~Closer() { q.close(); } } closer{seed_q};  // no more seeds
         while (true) {
             int b;
             if (!free_seeds.pop(b)) return;
@@ -418,6 +426,32 @@ int main(int argc, char** argv) {
         }
     });
 
+    // ---- output verification: SHA3-256 of each output on verify_ampout_threads workers (the
+    // reference's verifyDataPool, PrivacyAmplification.cu:1160-1180, 1130-1140), off the send path.
+    // An output buffer returns to the free list once its worker has hashed it.
+    struct VerifyJob {
+        int out_buf;
+        size_t bytes;
+    };
+    Queue<VerifyJob> verify_q;
+    std::vector<std::thread> verifiers;
+    if (cfg.verify_ampout)
+        for (uint32_t i = 0; i < std::max(1u, cfg.verify_ampout_threads); ++i)
+            verifiers.emplace_back([&] {
+                VerifyJob j;
+                while (verify_q.pop(j)) {
+                    uint8_t dg[32];
+                    pa_sha3::sha3_256(outs[j.out_buf], j.bytes, dg);
+                    if (std::memcmp(dg, expect, 32) != 0) {
+                        say("VERIFICATION FAILED!\n");
+                        set_fatal(101);
+                    } else {
+                        say("VERIFIED!\n");
+                    }
+                    free_outs.push(j.out_buf);
+                }
+            });
+
     // ---- outputs, in block order
     zmtp::Endpoint* out_ep = cfg.host_ampout_server ? new zmtp::Endpoint("PUSH", cfg.address_amp_out, true) : nullptr;
     std::thread tx([&] {
@@ -427,6 +461,10 @@ int main(int argc, char** argv) {
         auto t_prev = std::chrono::steady_clock::now();
         Done d;
         while (done_q.pop(d)) {
+            {
+                std::lock_guard<std::mutex> lk(fatal_mu);
+                if (fatal) break;
+            }
             if (d.ticket) {
                 uint64_t lat = 0;
                 const int st = pa_serve_wait(eng, d.ticket, -1, &lat);
@@ -437,22 +475,15 @@ int main(int argc, char** argv) {
                 }
                 const size_t bytes = (d.flags & PA_COMPRESS) ? d.m / 8 : d.n / 16;
                 const uint8_t* o = outs[d.out_buf];
-                if (out_ep && !out_ep->send(zmtp::Message{zmtp::Frame(o, o + bytes)})) break;
+                if (out_ep && !out_ep->send(zmtp::Message{zmtp::Frame(o, o + bytes)})) {
+                    set_fatal(1);
+                    break;
+                }
                 if (out_ep && cfg.show_zeromq_status) say("Block sent to AMPOUT Client\n");
                 if (to_store != 0 && ampout) {
                     ampout.write(reinterpret_cast<const char*>(o), std::streamsize(bytes));
                     ampout.flush();
                     if (to_store > 0 && --to_store == 0) ampout.close();
-                }
-                if (cfg.verify_ampout) {
-                    uint8_t dg[32];
-                    pa_sha3::sha3_256(o, bytes, dg);
-                    if (std::memcmp(dg, expect, 32) != 0) {
-                        say("VERIFICATION FAILED!\n");
-                        set_fatal(101);
-                        break;
-                    }
-                    say("VERIFIED!\n");
                 }
                 const auto now = std::chrono::steady_clock::now();
                 const double ms = std::chrono::duration<double, std::milli>(now - t_prev).count();
@@ -465,16 +496,21 @@ int main(int argc, char** argv) {
                         say("0x%02X: %s\n", o[i], bin);
                     }
                 }
-                free_outs.push(d.out_buf);
+                if (cfg.verify_ampout) verify_q.push(VerifyJob{d.out_buf, bytes});  // the worker frees it
+                else free_outs.push(d.out_buf);
                 free_keys.push(d.key_buf);
             }
             if (d.seed_buf >= 0) free_seeds.push(d.seed_buf);
             if (d.drop_seed) pa_seed_drop(eng, d.drop_seed);
             if (d.last) break;
         }
+        verify_q.close();
+        for (auto& t : verifiers) t.join();
         free_keys.close();
         free_seeds.close();
         free_outs.close();
+        key_q.close();
+        seed_q.close();
     });
 
     // ---- blocks (the mainloop): seed choice, submission

@@ -145,6 +145,9 @@ def load_library():
                                  ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]
     L.pa_workspace_bytes.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
     L.pa_workspace_bytes.restype = ctypes.c_uint64
+    L.pa_block_extents.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                   ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint64),
+                                   ctypes.POINTER(ctypes.c_uint64)]
     L.pa_synth_bits.argtypes = [ctypes.c_uint64, ctypes.c_double, ctypes.c_uint64, u32p]
     L.pa_strerror.argtypes = [ctypes.c_int]
     L.pa_strerror.restype = ctypes.c_char_p
@@ -155,7 +158,7 @@ def load_library():
                  "pa_batch_device", "pa_run_batch", "pa_serve_start", "pa_serve_submit",
                  "pa_serve_submit_many",
                  "pa_serve_wait", "pa_serve_stop", "pa_host_alloc", "pa_host_free",
-                 "pa_block_device_split", "pa_run_block_split"):
+                 "pa_block_device_split", "pa_run_block_split", "pa_block_extents"):
         getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L
@@ -186,6 +189,20 @@ def workspace_bytes(n, m, flags=DEFAULT_FLAGS):
 
 def out_bytes(n, m, do_compress=True):
     return m // 8 if do_compress else n // 16
+
+
+BUFFERS = ("key", "seed", "out", "spec_key", "spec_seed", "bits_key", "bits_seed", "nib", "counts")
+
+
+def block_extents(n, m, flags=DEFAULT_FLAGS):
+    """pa_block_extents: (L, touched, alloc) for a block shape, touched / alloc as {buffer: bytes}
+    (bytes its kernels index at most / bytes every allocation path gives it).  Host only."""
+    L = load_library()
+    tl = ctypes.c_uint32()
+    t = (ctypes.c_uint64 * len(BUFFERS))()
+    a = (ctypes.c_uint64 * len(BUFFERS))()
+    _check(L.pa_block_extents(n, m, flags, ctypes.byref(tl), t, a), "pa_block_extents")
+    return tl.value, dict(zip(BUFFERS, list(t))), dict(zip(BUFFERS, list(a)))
 
 
 class Engine:

@@ -16,6 +16,7 @@ import time
 ROOT = pathlib.Path(__file__).resolve().parents[2]
 sys.path.insert(0, str(ROOT))
 from oracle import oracle as O  # noqa: E402
+from privacyamplification_amd import workloads as W  # noqa: E402
 
 
 def entry(name, n, m, key_seed, toeplitz_seed, p_key=0.5):
@@ -30,7 +31,10 @@ def entry(name, n, m, key_seed, toeplitz_seed, p_key=0.5):
 
 
 def plan():
-    """(name, n, m, key_seed, toeplitz_seed, p_key) of every manifest block."""
+    """(name, n, m, key_seed, toeplitz_seed, p_key) of every manifest block: the single-block
+    configurations, the 256 uneven blocks of the batched workload (c4_*), the blocks a serving
+    stream draws from (c5_s<stream>_<key>, c5_d<index>; privacyamplification_amd/workloads.py) and
+    the 2^25-bit blocks of the serving DMA test (dma_*)."""
     items = [("pr1_n20", 1 << 20, 3 << 17, 0x11, 0x22, 0.5)]
     n = 1 << 27
     m = 3 * n // 8
@@ -42,6 +46,16 @@ def plan():
         nn = 1 << e
         for pk in (0.5, 0.9):
             items.append((f"c3_n{e}_p{pk}", nn, 3 * nn // 8, 0x3000 + e, 0x4000 + e, pk))
+    for i, (n, m, ks, ts) in enumerate(W.batched_mix()):
+        items.append((f"c4_{i}", n, m, ks, ts, 0.5))
+    streams, dynamic = W.serving_set()
+    for k, (n, m, ts, kss) in enumerate(streams):
+        for j, ks in enumerate(kss):
+            items.append((f"c5_s{k}_{j}", n, m, ks, ts, 0.5))
+    for j, (n, m, ks, ts) in enumerate(dynamic):
+        items.append((f"c5_d{j}", n, m, ks, ts, 0.5))
+    for i in (2, 3):  # tests/test_gpu_serve.py::test_serve_large_blocks_by_dma
+        items.append((f"dma_{i}", 1 << 25, (7 << 22) if i == 2 else (3 << 22), 0xD100 + i, 0xD200 + i, 0.5))
     return items
 
 

@@ -327,3 +327,34 @@ def test_linearity_at_full_size(engine, oracle):
     b = oracle.synth_bits(0xB, n + 1)
     oa, ob = engine.amplify(a, seed, n, m), engine.amplify(b, seed, n, m)
     assert np.array_equal(engine.amplify(a ^ b, seed, n, m), oa ^ ob)
+
+
+CANARY_SHAPES = [(1 << 10, 1 << 9, 3), (1 << 14, 6144, 3), (1 << 15, 7 << 12, 2), (1 << 15, 3 << 12, 0),
+                 (1 << 20, (1 << 20) - 32, 2), (1 << 20, 3 << 17, 1), (96000, 24000, 3), (3 << 20, 7 << 18, 2),
+                 (1 << 24, 32, 3), (1 << 24, 7 << 21, 0), ((1 << 25) + 32, 5 << 20, 2)]
+
+
+@pytest.mark.parametrize("n,m,flags", CANARY_SHAPES)
+def test_output_writes_stay_in_window(engine, oracle, n, m, flags):
+    """The output stage writes exactly the pa_block_extents window (m/8 bytes with compression,
+    n/16 without): a canary past it survives, and the window equals the oracle."""
+    import torch
+    import privacyamplification_amd as P
+    do_xor, do_compress = bool(flags & 1), bool(flags & 2)
+    _, touched, _ = P.block_extents(n, m, flags)
+    nb = touched["out"]
+    assert nb == P.out_bytes(n, m, do_compress)
+    say(f"canary n={n} m={m} flags={flags}")
+    key = oracle.synth_bits(0x600 + n % 977, n + 1)
+    seed = oracle.synth_bits(0x700 + n % 977, n)
+    if not do_xor:
+        key = key[: (n - m) // 32 + 1].copy()
+    dev = torch.device("cuda:0")
+    dk = torch.from_numpy(np.concatenate([key, np.zeros(2, np.uint32)]).view(np.int32)).to(dev)
+    ds = torch.from_numpy(seed.view(np.int32)).to(dev)
+    out = torch.full((nb + 512,), 0xA5, dtype=torch.uint8, device=dev)
+    engine.block_device(dk.data_ptr(), ds.data_ptr(), n, m, flags, out.data_ptr())
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    assert np.all(got[nb:] == 0xA5), np.nonzero(got[nb:] != 0xA5)[0][:8]
+    assert np.array_equal(got[:nb], oracle.amp_out(key, seed, n, m, do_xor, do_compress, method=oracle.NTT))

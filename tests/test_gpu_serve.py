@@ -1,6 +1,9 @@
 """Serving (pa_serve_*): blocks submitted while earlier ones are on the GPU are admitted into
 batches as slots and HBM budget free up; every result must equal the block computed alone and the
 oracle.  Runs on an MI355X (marker gpu)."""
+import hashlib
+import json
+import pathlib
 import time
 
 import numpy as np
@@ -9,6 +12,7 @@ import pytest
 import privacyamplification_amd as P
 
 pytestmark = pytest.mark.gpu
+MANIFEST = json.loads((pathlib.Path(__file__).resolve().parent / "golden" / "manifest.json").read_text())["blocks"]
 
 
 def say(*a):
@@ -156,11 +160,13 @@ def test_serve_large_blocks_by_dma(engine, oracle):
             reqs.append((k, n, m, out, sd, sid if resident else 0))
             meta.append((out, key, seed, n, m))
         tickets = engine.serve_submit_many(reqs)
-        for t, (out, key, seed, n, m) in zip(tickets, meta):
+        digests = {(b["n"], b["m"], b["key_seed"]): b["sha3"] for b in MANIFEST if b["name"].startswith("dma_")}
+        for i, (t, (out, key, seed, n, m)) in enumerate(zip(tickets, meta)):
             engine.serve_wait(t, 120 * 10**9)
-            assert np.array_equal(out, engine.amplify(key, seed, n, m)), (n, m)
             if n <= 1 << 23:
                 assert np.array_equal(out, _ref(oracle, key, seed, n, m)), (n, m)
+            else:  # 2^25-bit blocks: the oracle's digests, committed (tests/golden/make_manifest.py)
+                assert hashlib.sha3_256(out.tobytes()).hexdigest() == digests[(n, m, 0xD100 + i)], (n, m)
     finally:
         engine.serve_stop()
         engine.seed_drop(sid)
@@ -226,3 +232,30 @@ def test_serve_budget_admission(engine, oracle):
         assert ei.value.status == -4
     finally:
         engine.serve_stop()
+
+
+def test_destroy_while_serving_drains_first(oracle):
+    """pa_destroy with blocks still queued stops the scheduler first: every queued and in-flight
+    block completes against the resident seeds, workspaces and tables before they are freed."""
+    eng = P.Engine()
+    n0, m0 = 1 << 18, 3 << 15
+    sid = eng.seed_put(oracle.synth_bits(0xDE0, n0), n0, m0)
+    eng.serve_start(slots=3, max_batch=8, hbm_budget_bytes=1 << 30)
+    jobs = []
+    say("destroy with 60 blocks queued")
+    for i in range(60):
+        if i % 3 == 0:
+            n, m, seed = n0, m0, None
+        else:
+            n = [1 << 16, 96000, 1 << 20][i % 3]
+            m = n // 4 // 32 * 32
+            seed = oracle.synth_bits(0xDE00 + i, n)
+        key = oracle.synth_bits(0xDF00 + i, n + 1)
+        out = np.zeros(m // 8 + 8, np.uint8)
+        eng.serve_submit(key, n, m, out, seed=seed, sid=0 if seed is not None else sid)
+        jobs.append((out, key, seed, n, m))
+    eng.close()  # no serve_stop, no waits: pa_destroy must drain the queue itself
+    s0 = oracle.synth_bits(0xDE0, n0)
+    for i, (out, key, seed, n, m) in enumerate(jobs):
+        ref = _ref(oracle, key, s0 if seed is None else seed, n, m)
+        assert np.array_equal(out[: m // 8], ref), i

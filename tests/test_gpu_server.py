@@ -144,3 +144,46 @@ def test_pa_server_speedtest(tmp_path):
     assert all(v > 0 for v in d.values())
     print("speedtest 2^27 Mbit/s: dynamic %.0f, resident %.0f" % (
         sorted(d[0, 27, k] for k in range(10))[5], sorted(d[1, 27, k] for k in range(10))[5]))
+
+
+def test_pa_server_verify_pool_2_24(tmp_path, oracle):
+    """verify_ampout on a pool of verify_ampout_threads workers (PrivacyAmplification.cu:1160-1180):
+    12 blocks of 2^24 bits from files, every output hashed off the send path against the right
+    digest; every block prints VERIFIED! and ampout.bin holds them all."""
+    import hashlib
+    exe = ROOT / "privacyamplification_amd" / "pa_server"
+    e = 24
+    n, m = 1 << e, 3 << 21
+    key = oracle.synth_bits(0xA5, n + 1)
+    seed = oracle.synth_bits(0xA6, n)
+    (tmp_path / "key.bin").write_bytes(key.tobytes())
+    (tmp_path / "seed.bin").write_bytes(seed.tobytes())
+    ref = oracle.amp_out(key, seed, n, m, method=oracle.NTT).tobytes()
+    digest = hashlib.sha3_256(ref).hexdigest()
+    print("[case] pa_server verify pool n=2^24 12 blocks 4 threads", flush=True)
+    _file_config(tmp_path, e, m, f"verify_ampout: true\nverify_ampout_threads: 4\nverify_ampout_sha3: '{digest}'\n"
+                                 "output_blocks_to_cache: 6")
+    r = subprocess.run([str(exe), "--blocks", "12"], cwd=tmp_path, capture_output=True, text=True, timeout=180)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr
+    assert r.stdout.count("VERIFIED!") == 12 and "FAILED" not in r.stdout
+    assert (tmp_path / "ampout.bin").read_bytes() == ref * 12
+
+
+def test_pa_server_verify_failure_ends_unbounded_run(tmp_path, oracle):
+    """Without --blocks the block loop runs until its inputs end; file inputs never end, so only
+    the fatal path stops it.  A wrong verify digest must end the process with code 101 (the
+    reference exits at once, PrivacyAmplification.cu:1136) instead of waiting for more keys."""
+    import hashlib
+    exe = ROOT / "privacyamplification_amd" / "pa_server"
+    e, m = 16, 3 << 13
+    n = 1 << e
+    key = oracle.synth_bits(0xA7, n + 1)
+    seed = oracle.synth_bits(0xA8, n)
+    (tmp_path / "key.bin").write_bytes(key.tobytes())
+    (tmp_path / "seed.bin").write_bytes(seed.tobytes())
+    digest = hashlib.sha3_256(b"not the output").hexdigest()
+    print("[case] pa_server unbounded run, wrong digest -> 101", flush=True)
+    _file_config(tmp_path, e, m, f"verify_ampout: true\nverify_ampout_sha3: '{digest}'\n"
+                                 "input_blocks_to_cache: 2\noutput_blocks_to_cache: 2\nstore_first_ampouts_in_file: 0")
+    r = subprocess.run([str(exe)], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 101 and "VERIFICATION FAILED!" in r.stdout, r.stdout[-2000:] + r.stderr
