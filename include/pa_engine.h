@@ -114,6 +114,17 @@ int pa_block_device_seeded(pa_engine* eng, const uint32_t* d_key, pa_seed_id see
                            uint32_t m, uint32_t flags, uint8_t* d_out, void* stream,
                            float* d_margin);
 
+/* pa_block_device that allocates nothing.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory -- code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc.  The block's scratch (spectra, bit masks, nibbles, popcount partials)
+ * lives in one file-scope device array of 64 MiB per GPU, held until the process ends; the twiddle
+ * tables it needs are made by pa_create.  Serves transform lengths up to 2^22 bits (n <= 2^22);
+ * larger blocks return PA_ERR_SIZE (pa_strerror names the limit).  Same arguments and output as
+ * pa_block_device.  Every call on a GPU shares that GPU's array and leaves its contents behind:
+ * calls that overlap on one GPU (other streams, other engines, other threads) race. */
+int pa_block_device_noalloc(pa_engine* eng, const uint32_t* d_key, const uint32_t* d_seed, uint32_t n,
+                            uint32_t m, uint32_t flags, uint8_t* d_out, void* stream, float* d_margin);
+
 /* One block of a batch.  For pa_batch_device every pointer is a device pointer; for pa_run_batch
  * key, seed and out are host pointers and margin must be NULL. */
 typedef struct {

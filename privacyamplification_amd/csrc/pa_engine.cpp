@@ -40,6 +40,7 @@ using pa::Geom;
     } while (0)
 
 thread_local char g_last_error[512] = "";
+thread_local char g_size_error[256] = "";  // detail of the last PA_ERR_SIZE that has one
 void set_error(hipError_t e, const char* what, const char* file, int line) {
     snprintf(g_last_error, sizeof(g_last_error), "%s: %s (%s:%d)", what, hipGetErrorString(e), file, line);
 }
@@ -54,6 +55,7 @@ uint32_t log2u(uint64_t x) {
 // Plans the transform for a block: see pa_geom.hpp.  Returns PA_OK or an error status.
 int make_geom(uint32_t n, uint32_t m, uint32_t flags, uint32_t reverse_endian, uint32_t max_n_log2,
               Geom* g) {
+    g_size_error[0] = '\0';
     if (flags & ~3u) return PA_ERR_ARG;
     if (n < 64 || (n % 32) || m == 0 || (m % 32) || m >= n) return PA_ERR_SIZE;
     const bool pow2 = is_pow2(n);
@@ -229,6 +231,32 @@ struct BatchSlot {
     std::map<std::string, hipGraphExec_t> graphs;
 };
 
+// ------------------------------------------------------------------ allocation-free scratch ----
+// pa_block_device_noalloc's device scratch: one array per GPU, held until the process ends.  In
+// host code `scratch` is only a stand-in for the symbol; its device address comes from
+// hipGetSymbolAddress, made once per device at pa_create with that device current, and only that
+// address is ever handed to kernels.
+constexpr uint32_t NOALLOC_MAX_L_LOG = 22;
+constexpr uint64_t NOALLOC_SCRATCH_BYTES = 64ull << 20;
+static __device__ __align__(256) uint8_t scratch[NOALLOC_SCRATCH_BYTES];
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// The device address of `scratch` on the current device `dev` (looked up on first use).
+hipError_t scratch_address(int dev, void** out) {
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t r = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (r != hipSuccess) return r;
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+
 }  // namespace
 
 struct pa_engine {
@@ -263,6 +291,7 @@ struct pa_engine {
     uint8_t* split = nullptr;                // sub-block mode: tile inputs/outputs, band accumulator,
     uint64_t split_cap = 0;                  // host-API staging (grow-only)
     std::mutex pin_mu;
+    uint8_t* noalloc = nullptr;              // this device's `scratch` address (pa_block_device_noalloc)
 };
 
 namespace {
@@ -622,6 +651,18 @@ int pa_create(const pa_config* cfg, pa_engine** out) {
     if (rc != PA_OK) return rc;
     auto cls = class_tables();
     PA_HIP(hipMemcpy(e->cls, cls.data(), cls.size() * sizeof(float2), hipMemcpyHostToDevice));
+    // pa_block_device_noalloc: the scratch array's address on this device, and the twiddle tables
+    // of every four-step length it serves, so that its calls allocate nothing
+    {
+        void* sp = nullptr;
+        PA_HIP(scratch_address(e->cfg.device, &sp));
+        e->noalloc = static_cast<uint8_t*>(sp);
+        for (uint32_t lg = pa::SMALL_MAX_LOG_N + 2; lg <= std::min(NOALLOC_MAX_L_LOG, e->cfg.max_n_log2); ++lg) {
+            Twiddles* tw;
+            rc = get_twiddles(e.get(), 1u << lg, &tw);
+            if (rc != PA_OK) return rc;
+        }
+    }
     *out = e.release();
     return PA_OK;
 }
@@ -774,6 +815,48 @@ int pa_block_device_seeded(pa_engine* e, const uint32_t* d_key, pa_seed_id id, u
     rc = seed_lookup(e, id, g, &sd);
     if (rc != PA_OK) return rc;
     return run_device(e, g, w, d_key, nullptr, sd->spec, sd->ws, (uint32_t*)d_out, stream_of(e, stream), d_margin);
+}
+
+int pa_block_device_noalloc(pa_engine* e, const uint32_t* d_key, const uint32_t* d_seed, uint32_t n, uint32_t m,
+                            uint32_t flags, uint8_t* d_out, void* stream, float* d_margin) {
+    if (!e || !d_key || !d_seed || !d_out) return PA_ERR_ARG;
+    std::lock_guard<std::mutex> lk(e->mu);
+    Geom g;
+    int rc = make_geom(n, m, flags, e->cfg.out_reverse_endian, e->cfg.max_n_log2, &g);
+    if (rc != PA_OK) return rc;
+    if (g.L > (1u << NOALLOC_MAX_L_LOG)) {
+        snprintf(g_size_error, sizeof(g_size_error),
+                 "pa_block_device_noalloc serves transform lengths up to 2^%u bits (n <= 2^%u); n = %u needs 2^%u",
+                 NOALLOC_MAX_L_LOG, NOALLOC_MAX_L_LOG, n, log2u(g.L));
+        return PA_ERR_SIZE;
+    }
+    if (!block_extents(g).ok()) return PA_ERR_UNSUPPORTED;
+    if (!e->noalloc || e->tw.find(g.L) == e->tw.end()) return PA_ERR_UNSUPPORTED;
+    // the workspace of this block, carved from the scratch array (no allocation); the popcount
+    // partials are cleared by a kernel at the start of the block (counts_zero = false)
+    Workspace w;
+    w.L = g.L;
+    if (g.N1 > 1) {
+        uint64_t off = 0;
+        auto take = [&](uint64_t bytes) {
+            uint8_t* q = e->noalloc + off;
+            off += (bytes + 255u) & ~uint64_t(255);
+            return q;
+        };
+        w.Yk = reinterpret_cast<float2*>(take((uint64_t)g.N * 8));
+        w.Ys = reinterpret_cast<float2*>(take((uint64_t)(g.N + 1) * 8));
+        w.tbk = reinterpret_cast<uint32_t*>(take((uint64_t)(g.N / g.D) * 4));
+        w.tbs = reinterpret_cast<uint32_t*>(take((uint64_t)(g.N / g.D) * 4));
+        w.nib = take(pa::nib_bytes(g.N, g.W));
+        w.counts = reinterpret_cast<uint32_t*>(take(2 * pa::POP_SLOTS * 4));
+        if (off > NOALLOC_SCRATCH_BYTES) return PA_ERR_UNSUPPORTED;
+    }
+    const bool timing = e->timing;  // stage timing creates events: not on this path
+    e->timing = false;
+    rc = run_device_body(e, g, &w, d_key, d_seed, nullptr, nullptr, reinterpret_cast<uint32_t*>(d_out),
+                         stream_of(e, stream), d_margin, nullptr);
+    e->timing = timing;
+    return rc;
 }
 
 static int run_host(pa_engine* e, const uint32_t* key, const uint32_t* seed, pa_seed_id id, uint32_t n,
@@ -1889,7 +1972,7 @@ const char* pa_strerror(int status) {
     switch (status) {
     case PA_OK: return "ok";
     case PA_ERR_ARG: return "invalid argument";
-    case PA_ERR_SIZE: return "block size outside the supported family";
+    case PA_ERR_SIZE: return g_size_error[0] ? g_size_error : "block size outside the supported family";
     case PA_ERR_DEVICE: return g_last_error[0] ? g_last_error : "HIP runtime error";
     case PA_ERR_NOMEM: return "device memory or HBM budget exhausted";
     case PA_ERR_TIMEOUT: return "timed out";

@@ -126,6 +126,8 @@ def load_library():
                                   vp, vp]
     L.pa_block_device_seeded.argtypes = [eng, vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
                                          ctypes.c_uint32, vp, vp, vp]
+    L.pa_block_device_noalloc.argtypes = [eng, vp, vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, vp,
+                                          vp, vp]
     L.pa_batch_device.argtypes = [eng, ctypes.POINTER(PaBlock), ctypes.c_uint32, ctypes.c_uint32, vp]
     L.pa_run_batch.argtypes = [eng, ctypes.POINTER(PaBlock), ctypes.c_uint32, ctypes.c_uint32]
     L.pa_block_device_split.argtypes = [eng, vp, vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
@@ -158,7 +160,8 @@ def load_library():
                  "pa_batch_device", "pa_run_batch", "pa_serve_start", "pa_serve_submit",
                  "pa_serve_submit_many",
                  "pa_serve_wait", "pa_serve_stop", "pa_host_alloc", "pa_host_free",
-                 "pa_block_device_split", "pa_run_block_split", "pa_block_extents"):
+                 "pa_block_device_split", "pa_run_block_split", "pa_block_extents",
+                 "pa_block_device_noalloc"):
         getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L
@@ -292,6 +295,11 @@ class Engine:
         """Device-resident block: integer device addresses (e.g. torch tensor data_ptr())."""
         _check(self._L.pa_block_device(self._h, d_key, d_seed, n, m, flags, d_out, stream, d_margin),
                "pa_block_device")
+
+    def block_device_noalloc(self, d_key, d_seed, n, m, flags, d_out, stream=None, d_margin=None):
+        """pa_block_device without any allocation (scratch in a fixed device array; n <= 2^22)."""
+        _check(self._L.pa_block_device_noalloc(self._h, d_key, d_seed, n, m, flags, d_out, stream, d_margin),
+               "pa_block_device_noalloc")
 
     def block_device_seeded(self, d_key, sid, n, m, flags, d_out, stream=None, d_margin=None):
         _check(self._L.pa_block_device_seeded(self._h, d_key, sid, n, m, flags, d_out, stream, d_margin),
