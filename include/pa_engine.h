@@ -49,7 +49,8 @@ enum pa_status {
     PA_ERR_NOMEM = -4,       /* device allocation failed or HBM budget exceeded */
     PA_ERR_TIMEOUT = -5,     /* pa_wait timed out */
     PA_ERR_NOT_FOUND = -6,   /* seed id / ticket unknown */
-    PA_ERR_UNSUPPORTED = -7  /* valid request this build does not serve */
+    PA_ERR_UNSUPPORTED = -7, /* valid request this build does not serve */
+    PA_ERR_CHECK = -8        /* a self-test (pa_selftest) found a wrong result */
 };
 
 /* Engine configuration.  The fields mirror config.yaml (PrivacyAmplification/config.yaml:6-124,
@@ -248,6 +249,16 @@ int pa_block_extents(uint32_t n, uint32_t m, uint32_t flags, uint32_t* transform
 /* Synthetic RefBits: xoshiro256** seeded through splitmix64, Bernoulli(p) bits; p == 0.5 takes
  * two words per 64-bit draw (high half first).  Host memory, nwords = ceil(nbits/32). */
 int pa_synth_bits(uint64_t seed, double p, uint64_t nbits, uint32_t* out);
+
+/* Device self-tests, the counterpart of the reference's unitTest* argv modes
+ * (PrivacyAmplification.cu:299-737, dispatched at :1645-1652).  name is the reference's test name
+ * without "unitTest": "CalculateCorrectionFloat", "SetFirstElementToZero", "ElementWiseProduct",
+ * "BinInt2float" or "ToBinaryArray".  Each runs this engine's device code for that step (its device
+ * functions or its production kernels) on the reference test's closed-form inputs and checks the
+ * result on the host.  report (may be NULL) receives a one-line summary or the first mismatch.
+ * Returns PA_OK when the test passed, PA_ERR_CHECK when it found a wrong value, PA_ERR_ARG for an
+ * unknown name. */
+int pa_selftest(pa_engine* eng, const char* name, char* report, uint32_t report_bytes);
 
 const char* pa_strerror(int status);
 const char* pa_version(void);

@@ -15,6 +15,7 @@ from .engine import (  # noqa: F401
     EngineError,
     PaBlock,
     PaServeConfig,
+    SELFTESTS,
     block_extents,
     build,
     library_path,

@@ -383,6 +383,22 @@ __device__ __forceinline__ void st_nt(float2* p, float2 v) {
 }
 __device__ __forceinline__ void st_nt_u8(uint8_t* p, uint8_t v) { __builtin_nontemporal_store(v, p); }
 
+// corr = (wk * ws / L) mod 2, exactly: the mean term that zeroing both DC bins removes
+// (calculateCorrectionFloat, PrivacyAmplification.cu:363-373, evaluated in 64-bit integers).
+__device__ __forceinline__ float correction(uint32_t wk, uint32_t ws, uint32_t L) {
+    const uint64_t prod = (uint64_t)wk * (uint64_t)ws;
+    const uint64_t r = prod & (2ull * L - 1ull);  // L is a power of two
+    return (float)((double)r / (double)L);
+}
+
+// The output bit of a pre-rounding value: round half to even, then the parity (ToBinaryArray,
+// PrivacyAmplification.cu:739-783; glsl/toBinaryArray.comp:48); dist tracks max |v - rint v|.
+__device__ __forceinline__ uint32_t round_bit(float v, float& dist) {
+    const float r = rintf(v);
+    dist = fmaxf(dist, fabsf(v - r));
+    return (uint32_t)__float2int_rn(v) & 1u;
+}
+
 // Wave-level max of a non-negative float published to a device float with an atomic max
 // on its bit pattern (non-negative IEEE floats order like their unsigned bit patterns).
 __device__ __forceinline__ void margin_publish(float v, float* margin) {

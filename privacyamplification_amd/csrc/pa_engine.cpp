@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <cstdarg>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -1968,6 +1969,298 @@ int pa_synth_bits(uint64_t seed, double p, uint64_t nbits, uint32_t* out) {
     return PA_OK;
 }
 
+}  // extern "C"
+
+// ------------------------------------------------------------------ self-tests ----
+// The counterparts of the reference's unitTest* argv modes (PrivacyAmplification.cu:299-737,
+// dispatched at :1645-1652).  The reference tests each of its five small kernels on closed-form
+// inputs; this engine fuses those steps into its passes, so each test runs the production device
+// code of that step -- the same device functions or the production kernels themselves -- on the
+// same closed-form inputs and checks the result on the host.
+namespace {
+
+struct DevMem {
+    void* p = nullptr;
+    ~DevMem() { if (p) (void)hipFree(p); }
+    template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+#define ST_ALLOC(buf, bytes) PA_HIP(hipMalloc(&(buf).p, (bytes)))
+
+struct Report {
+    char* out;
+    uint32_t cap;
+    bool ok = true;
+    void fail(const char* fmt, ...) __attribute__((format(printf, 2, 3))) {
+        if (ok && out && cap) {
+            va_list ap;
+            va_start(ap, fmt);
+            vsnprintf(out, cap, fmt, ap);
+            va_end(ap);
+        }
+        ok = false;
+    }
+    int done(const char* what) {
+        if (ok && out && cap) snprintf(out, cap, "%s: passed", what);
+        return ok ? PA_OK : PA_ERR_CHECK;
+    }
+};
+
+// Ones among the binary digits of 0, 1, ..., x (OEIS A000788, the popcount closed form of
+// unitTestBinInt2float): per bit b, full periods of 2^(b+1) contribute 2^b each, and the last
+// partial period max(0, rem - 2^b).
+uint64_t ones_upto(uint64_t x) {
+    uint64_t v = 0;
+    for (int b = 0; b < 40; ++b) {
+        const uint64_t period = 2ull << b, half = 1ull << b;
+        const uint64_t full = (x + 1) / period, rem = (x + 1) % period;
+        v += full * half + (rem > half ? rem - half : 0);
+    }
+    return v;
+}
+
+int st_correction(pa_engine* e, Report& rp) {
+    // i, j < 64 with L = 64, then 4096 pairs in [2^25, 2^27] with L = 2^27 (:299-361)
+    std::vector<uint32_t> wk, ws, Ls;
+    for (uint32_t i = 0; i < 64; ++i)
+        for (uint32_t j = 0; j < 64; ++j) { ws.push_back(i); wk.push_back(j); Ls.push_back(64); }
+    uint64_t x = 777;
+    for (int k = 0; k < 2 * 4096; ++k) {
+        x = x * 6364136223846793005ull + 1442695040888963407ull;
+        const uint32_t v = (1u << 25) + (uint32_t)((x >> 33) % ((1u << 27) - (1u << 25) + 1u));
+        if (k & 1) { wk.push_back(v); Ls.push_back(1u << 27); } else ws.push_back(v);
+    }
+    const uint32_t cnt = (uint32_t)wk.size();
+    DevMem dwk, dws, dout;
+    ST_ALLOC(dwk, cnt * 4);
+    ST_ALLOC(dws, cnt * 4);
+    ST_ALLOC(dout, cnt * 4);
+    PA_HIP(hipMemcpy(dwk.p, wk.data(), cnt * 4, hipMemcpyHostToDevice));
+    PA_HIP(hipMemcpy(dws.p, ws.data(), cnt * 4, hipMemcpyHostToDevice));
+    std::vector<float> got(cnt);
+    for (uint32_t lo = 0; lo < cnt;) {  // one launch per L
+        uint32_t hi = lo;
+        while (hi < cnt && Ls[hi] == Ls[lo]) ++hi;
+        PA_HIP(pa::launch_st_correction(dwk.as<uint32_t>() + lo, dws.as<uint32_t>() + lo, Ls[lo], dout.as<float>() + lo,
+                                        hi - lo, e->stream));
+        lo = hi;
+    }
+    PA_HIP(hipMemcpyAsync(got.data(), dout.p, cnt * 4, hipMemcpyDeviceToHost, e->stream));
+    PA_HIP(hipStreamSynchronize(e->stream));
+    for (uint32_t k = 0; k < cnt; ++k) {
+        // the reference's host expectation: the 32-bit product, / n, mod 2
+        const double want = std::fmod((double)(uint32_t)(wk[k] * ws[k]) / (double)Ls[k], 2.0);
+        if (std::fabs(got[k] - want) > 1e-4) {
+            rp.fail("CalculateCorrectionFloat: case %u (wk %u, ws %u, L %u): %.7f, want %.7f", k, wk[k], ws[k], Ls[k],
+                    got[k], want);
+            break;
+        }
+    }
+    return rp.done("CalculateCorrectionFloat");
+}
+
+// The seed's split spectrum through the production pass A and row pass (pa_seed_put's launches).
+int st_seed_spectrum(pa_engine* e, const uint32_t* h_seed, uint32_t n, std::vector<float2>& S, uint32_t& ws) {
+    Geom g;
+    Workspace* w;
+    int rc = prepare(e, n, 3 * n / 8, PA_DEFAULT_FLAGS, &g, &w);
+    if (rc != PA_OK) return rc;
+    DevMem dS, dws, dseed;
+    ST_ALLOC(dS, (size_t)(g.N + 1) * 8);
+    ST_ALLOC(dws, pa::POP_SLOTS * 4);
+    ST_ALLOC(dseed, (size_t)n / 8);
+    hipStream_t s = e->stream;
+    PA_HIP(hipMemcpyAsync(dseed.p, h_seed, n / 8, hipMemcpyHostToDevice, s));
+    const pa::Tables t = tables(e, g.L);
+    PA_HIP(pa::launch_zero_u32(dws.as<uint32_t>(), pa::POP_SLOTS, s));
+    PA_HIP(pa::launch_transpose(dseed.as<uint32_t>(), w->tbs, dws.as<uint32_t>(), g, 1, s));
+    PA_HIP(pa::launch_fwd_cols(w->tbs, dS.as<float2>(), g, t, s));
+    PA_HIP(pa::launch_rows(pa::ROWS_SEEDPREP, nullptr, dS.as<float2>(), g, t, e->cus, s));
+    S.resize(g.N + 1);
+    std::vector<uint32_t> part(pa::POP_SLOTS);
+    PA_HIP(hipMemcpyAsync(S.data(), dS.p, (size_t)(g.N + 1) * 8, hipMemcpyDeviceToHost, s));
+    PA_HIP(hipMemcpyAsync(part.data(), dws.p, pa::POP_SLOTS * 4, hipMemcpyDeviceToHost, s));
+    PA_HIP(hipStreamSynchronize(s));
+    ws = 0;
+    for (uint32_t v : part) ws += v;
+    return PA_OK;
+}
+
+int st_dc_zero(pa_engine* e, Report& rp) {
+    // setFirstElementToZero (:377-418): the DC bin of the seed spectrum leaves the row pass as an
+    // exact zero; an all-ones seed (a constant sequence, spectrum at DC only) leaves nothing else
+    const uint32_t n = 1u << 20;
+    std::vector<uint32_t> ones(n / 32, 0xFFFFFFFFu), rnd(n / 32);
+    pa_synth_bits(0x5E1F, 0.5, n, rnd.data());
+    for (int which = 0; which < 2 && rp.ok; ++which) {
+        std::vector<float2> S;
+        uint32_t ws = 0;
+        int rc = st_seed_spectrum(e, which ? rnd.data() : ones.data(), n, S, ws);
+        if (rc != PA_OK) return rc;
+        uint32_t want_ws = 0;
+        for (uint32_t v : (which ? rnd : ones)) want_ws += (uint32_t)__builtin_popcount(v);
+        if (S[0].x != 0.f || S[0].y != 0.f) rp.fail("SetFirstElementToZero: DC bin (%g, %g), want 0", S[0].x, S[0].y);
+        else if (ws != want_ws) rp.fail("SetFirstElementToZero: seed popcount %u, want %u", ws, want_ws);
+        double mx = 0.0;
+        for (size_t k = 1; k < S.size(); ++k) mx = std::max(mx, (double)std::hypot(S[k].x, S[k].y));
+        if (rp.ok && which == 0 && mx > 1e-4 * n)
+            rp.fail("SetFirstElementToZero: constant seed leaves |S| %.3g outside DC", mx);
+        if (rp.ok && which == 1 && mx < 1.0) rp.fail("SetFirstElementToZero: random seed spectrum vanished");
+    }
+    return rp.done("SetFirstElementToZero");
+}
+
+int st_product(pa_engine* e, Report& rp) {
+    // ElementWiseProduct (:422-481): do1 = i + 0.77, do2 = i + 0.88 over 2^10 complex values, r = 2^5
+    const uint32_t N = 1024;
+    const float r = 32.f;
+    std::vector<float2> a(N), b(N), z(N), prod(N), zz(N);
+    for (uint32_t i = 0; i < N; ++i) {
+        a[i] = make_float2(2 * i + 0.77f, 2 * i + 1 + 0.77f);
+        b[i] = make_float2(2 * i + 0.88f, 2 * i + 1 + 0.88f);
+        z[i] = make_float2(std::sin(0.37 * i + 0.1), std::cos(1.13 * i + 0.2));
+    }
+    DevMem da, db, dz, dp, dzz;
+    for (DevMem* d : {&da, &db, &dz, &dp, &dzz}) ST_ALLOC(*d, N * 8);
+    PA_HIP(hipMemcpy(da.p, a.data(), N * 8, hipMemcpyHostToDevice));
+    PA_HIP(hipMemcpy(db.p, b.data(), N * 8, hipMemcpyHostToDevice));
+    PA_HIP(hipMemcpy(dz.p, z.data(), N * 8, hipMemcpyHostToDevice));
+    PA_HIP(pa::launch_st_product(da.as<float2>(), db.as<float2>(), r, dp.as<float2>(), dz.as<float2>(),
+                                 dzz.as<float2>(), N, e->stream));
+    PA_HIP(hipMemcpyAsync(prod.data(), dp.p, N * 8, hipMemcpyDeviceToHost, e->stream));
+    PA_HIP(hipMemcpyAsync(zz.data(), dzz.p, N * 8, hipMemcpyDeviceToHost, e->stream));
+    PA_HIP(hipStreamSynchronize(e->stream));
+    for (uint32_t i = 0; i < N && rp.ok; ++i) {
+        const double ar = (2.0 * i + 0.77) / r, ai = (2.0 * i + 1 + 0.77) / r;
+        const double br = (2.0 * i + 0.88) / r, bi = (2.0 * i + 1 + 0.88) / r;
+        const double re = ar * br - ai * bi, im = ar * bi + ai * br;
+        if (std::fabs(prod[i].x - re) > 1e-3 || std::fabs(prod[i].y - im) > 1e-3)
+            rp.fail("ElementWiseProduct: value %u (%.6f, %.6f), want (%.6f, %.6f)", i, prod[i].x, prod[i].y, re, im);
+    }
+    for (uint32_t i = 1; i < N && rp.ok; ++i) {
+        if (i == N / 2) continue;
+        if (std::hypot(zz[i].x - 4 * z[i].x, zz[i].y - 4 * z[i].y) > 1e-5)
+            rp.fail("ElementWiseProduct: split round trip at %u (%.7f, %.7f), want 4 z = (%.7f, %.7f)", i, zz[i].x,
+                    zz[i].y, 4 * z[i].x, 4 * z[i].y);
+    }
+    return rp.done("ElementWiseProduct");
+}
+
+inline uint32_t refbit(const uint32_t* w, uint64_t p) { return (w[p >> 5] >> (31u - (p & 31u))) & 1u; }
+
+int st_unpack(pa_engine* e, Report& rp) {
+    // binInt2float (:484-602): binIn[i] = i; the popcount of n bits is A000788(n/32 - 1); here the
+    // production bit transpose of pass A also lays the bits out for the class tables, checked too
+    const uint32_t nmax = 1u << 27;
+    std::vector<uint32_t> bin(nmax / 32);
+    for (uint32_t i = 0; i < nmax / 32; ++i) bin[i] = i;
+    DevMem dbin, dcnt;
+    ST_ALLOC(dbin, nmax / 8);
+    ST_ALLOC(dcnt, pa::POP_SLOTS * 4);
+    PA_HIP(hipMemcpy(dbin.p, bin.data(), nmax / 8, hipMemcpyHostToDevice));
+    uint64_t lcg = 12345;
+    for (uint32_t lg = pa::SMALL_MAX_LOG_N + 2; lg <= 27 && rp.ok; ++lg) {
+        const uint32_t n = 1u << lg;
+        Geom g;
+        Workspace* w;
+        int rc = prepare(e, n, 3 * n / 8, PA_DEFAULT_FLAGS, &g, &w);
+        if (rc != PA_OK) return rc;
+        PA_HIP(pa::launch_zero_u32(dcnt.as<uint32_t>(), pa::POP_SLOTS, e->stream));
+        PA_HIP(pa::launch_transpose(dbin.as<uint32_t>(), w->tbs, dcnt.as<uint32_t>(), g, 1, e->stream));
+        std::vector<uint32_t> part(pa::POP_SLOTS), tb(g.N / g.D);
+        PA_HIP(hipMemcpyAsync(part.data(), dcnt.p, pa::POP_SLOTS * 4, hipMemcpyDeviceToHost, e->stream));
+        PA_HIP(hipMemcpyAsync(tb.data(), w->tbs, tb.size() * 4, hipMemcpyDeviceToHost, e->stream));
+        PA_HIP(hipStreamSynchronize(e->stream));
+        uint64_t cnt = 0;
+        for (uint32_t v : part) cnt += v;
+        if (cnt != ones_upto(n / 32 - 1)) {
+            rp.fail("BinInt2float: n = 2^%u popcount %llu, want %llu", lg, (unsigned long long)cnt,
+                    (unsigned long long)ones_upto(n / 32 - 1));
+            break;
+        }
+        // tb[u N2 + t2] = E | O << 16: bit v of E / O = seed bit 2(t2 + N2 (u + M v)) / that + 1
+        const uint64_t words = tb.size(), checks = lg <= 21 ? words : 65536;
+        for (uint64_t c = 0; c < checks && rp.ok; ++c) {
+            uint64_t idx = c;
+            if (checks != words) {
+                lcg = lcg * 6364136223846793005ull + 1442695040888963407ull;
+                idx = (lcg >> 20) % words;
+            }
+            const uint64_t u = idx / g.N2, t2 = idx % g.N2;
+            uint32_t want = 0;
+            for (uint32_t v = 0; v < g.D; ++v) {
+                const uint64_t p = 2 * (t2 + (uint64_t)g.N2 * (u + (uint64_t)g.M * v));
+                want |= refbit(bin.data(), p) << v;
+                want |= refbit(bin.data(), p + 1) << (16 + v);
+            }
+            if (tb[idx] != want) rp.fail("BinInt2float: n = 2^%u bit word %llu = %08x, want %08x", lg,
+                                         (unsigned long long)idx, tb[idx], want);
+        }
+    }
+    return rp.done("BinInt2float");
+}
+
+int st_epilogue(pa_engine* e, Report& rp) {
+    // ToBinaryArray (:605-737): v[p] = bit 31 - p % 32 of p / 32, corr = 1.9, norm = 1, key rest
+    // 0xAAAAAAAA words; output bit p = v[p] ^ key-rest bit p (NoXOR: v[p]), n = 2^10 .. 2^27,
+    // m = n/4 + n/8.  Pass A''s rounding (round_bit) writes the nibbles, the production pack stage
+    // assembles, XORs and orders the words.
+    const uint32_t nmax = 1u << 27;
+    DevMem dkey, dnib, dout;
+    ST_ALLOC(dkey, (nmax / 32 + 2) * 4);
+    ST_ALLOC(dout, nmax / 8 + 64);
+    ST_ALLOC(dnib, (size_t)nmax / 2 + 64);
+    {
+        // key words 0x55555555: key rest word w = (key[h/32+w] << 1) | (key[h/32+w+1] >> 31) = 0xAAAAAAAA
+        std::vector<uint32_t> key(nmax / 32 + 2, 0x55555555u);
+        PA_HIP(hipMemcpy(dkey.p, key.data(), key.size() * 4, hipMemcpyHostToDevice));
+    }
+    for (uint32_t lg = pa::SMALL_MAX_LOG_N + 2; lg <= 27 && rp.ok; ++lg) {
+        const uint32_t n = 1u << lg, m = n / 4 + n / 8;
+        for (int do_xor = 1; do_xor >= 0 && rp.ok; --do_xor) {
+            Geom g;
+            int rc = make_geom(n, m, PA_COMPRESS | (do_xor ? PA_XOR_KEY_REST : 0u), e->cfg.out_reverse_endian,
+                               e->cfg.max_n_log2, &g);
+            if (rc != PA_OK) return rc;
+            PA_HIP(pa::launch_st_nibbles(dnib.as<uint8_t>(), g, 1.9f, 1.0f, e->stream));
+            PA_HIP(pa::launch_pack(dnib.as<uint8_t>(), dkey.as<uint32_t>(), dout.as<uint32_t>(), g, e->stream));
+            std::vector<uint8_t> out(m / 8);
+            PA_HIP(hipMemcpyAsync(out.data(), dout.p, m / 8, hipMemcpyDeviceToHost, e->stream));
+            PA_HIP(hipStreamSynchronize(e->stream));
+            for (uint32_t p = 0; p < m; ++p) {
+                const uint32_t v = ((p >> 5) >> (31u - (p & 31u))) & 1u;
+                const uint32_t want = v ^ (do_xor ? (0xAAAAAAAAu >> (31u - (p & 31u))) & 1u : 0u);
+                uint32_t got;
+                if (g.reverse_endian) got = (out[p >> 3] >> (7u - (p & 7u))) & 1u;
+                else got = refbit(reinterpret_cast<const uint32_t*>(out.data()), p);
+                if (got != want) {
+                    rp.fail("ToBinaryArray%s: n = 2^%u output bit %u = %u, want %u", do_xor ? "" : "NoXOR", lg, p, got, want);
+                    break;
+                }
+            }
+        }
+    }
+    return rp.done("ToBinaryArray");
+}
+
+}  // namespace
+
+extern "C" {
+
+int pa_selftest(pa_engine* e, const char* name, char* report, uint32_t report_bytes) {
+    if (!e || !name) return PA_ERR_ARG;
+    if (report && report_bytes) report[0] = '\0';
+    std::lock_guard<std::mutex> lk(e->mu);
+    PA_HIP(hipSetDevice(e->cfg.device));
+    Report rp{report, report_bytes};
+    const std::string t = name;
+    if (t == "CalculateCorrectionFloat") return st_correction(e, rp);
+    if (t == "SetFirstElementToZero") return st_dc_zero(e, rp);
+    if (t == "ElementWiseProduct") return st_product(e, rp);
+    if (t == "BinInt2float") return st_unpack(e, rp);
+    if (t == "ToBinaryArray") return st_epilogue(e, rp);
+    return PA_ERR_ARG;
+}
+
 const char* pa_strerror(int status) {
     switch (status) {
     case PA_OK: return "ok";
@@ -1978,6 +2271,7 @@ const char* pa_strerror(int status) {
     case PA_ERR_TIMEOUT: return "timed out";
     case PA_ERR_NOT_FOUND: return "unknown seed or ticket";
     case PA_ERR_UNSUPPORTED: return "unsupported request";
+    case PA_ERR_CHECK: return "self-test failed";
     default: return "unknown status";
     }
 }

@@ -58,19 +58,6 @@ __device__ __forceinline__ uint32_t pop_total(const uint32_t* __restrict__ c) {
     return t;
 }
 
-// corr = (wk * ws / L) mod 2, exactly: the mean term that zeroing both DC bins removes
-// (calculateCorrectionFloat, PrivacyAmplification.cu:363-373, evaluated in 64-bit integers).
-__device__ __forceinline__ float correction(uint32_t wk, uint32_t ws, uint32_t L) {
-    const uint64_t prod = (uint64_t)wk * (uint64_t)ws;
-    const uint64_t r = prod & (2ull * L - 1ull);  // L is a power of two
-    return (float)((double)r / (double)L);
-}
-
-__device__ __forceinline__ uint32_t round_bit(float v, float& dist) {
-    const float r = rintf(v);
-    dist = fmaxf(dist, fabsf(v - r));
-    return (uint32_t)__float2int_rn(v) & 1u;
-}
 
 // Batched launches (BATCH = true) take each block's geometry and buffers from bt[blockIdx.z].
 template <bool BATCH>

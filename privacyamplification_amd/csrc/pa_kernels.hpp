@@ -104,4 +104,11 @@ hipError_t launch_tile_xor(uint32_t* acc, const uint32_t* tout, uint32_t nbands,
 hipError_t launch_split_finish(const uint32_t* acc, const uint32_t* key, uint32_t* out, uint32_t m, uint32_t h,
                                uint32_t do_xor, uint32_t reverse_endian, hipStream_t s);
 
+// Self-test kernels (pa_selftest.hip): production device functions on closed-form inputs.
+hipError_t launch_st_correction(const uint32_t* wk, const uint32_t* ws, uint32_t L, float* out, uint32_t count,
+                                hipStream_t s);
+hipError_t launch_st_product(const float2* a, const float2* b, float r, float2* prod, const float2* z, float2* zz,
+                             uint32_t N, hipStream_t s);
+hipError_t launch_st_nibbles(uint8_t* nib, const Geom& g, float corr, float invN, hipStream_t s);
+
 }  // namespace pa

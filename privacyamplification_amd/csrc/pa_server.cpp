@@ -18,7 +18,7 @@
 // Blocks go through the engine's serving scheduler (pa_serve_*) from pinned buffers
 // (pa_host_alloc); resident seeds (pa_seed_put) serve reused seeds; outputs leave in block order.
 //
-//   pa_server [--config config.yaml] [--factor_exp N] [--blocks K] [speedtest]
+//   pa_server [--config config.yaml] [--factor_exp N] [--blocks K] [speedtest | unitTest<Name>]
 // (--factor_exp overrides the config; --blocks K ends the process after K blocks.)
 #include <algorithm>
 #include <chrono>
@@ -266,12 +266,14 @@ int main(int argc, char** argv) {
     int64_t max_blocks = -1;
     int factor_override = -1;
     bool speedtest = false;
+    std::string unit_test;  // argv unitTest<Name> (PrivacyAmplification.cu:1645-1652)
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
         if (a == "--config" && i + 1 < argc) cfg_path = argv[++i];
         else if (a == "--factor_exp" && i + 1 < argc) factor_override = std::atoi(argv[++i]);
         else if (a == "--blocks" && i + 1 < argc) max_blocks = std::atoll(argv[++i]);
         else if (a == "speedtest") speedtest = true;
+        else if (a.compare(0, 8, "unitTest") == 0) unit_test = a.substr(8);
     }
     Config cfg;
     std::string err;
@@ -309,6 +311,19 @@ int main(int argc, char** argv) {
         const int code = run_speedtest(eng);
         pa_destroy(eng);
         return code;
+    }
+    if (!unit_test.empty()) {  // the engine's self-test of that step; 100 on failure, as the reference
+        say("Started %s Unit Test...\n", unit_test.c_str());
+        char report[512];
+        const int st = pa_selftest(eng, unit_test.c_str(), report, sizeof(report));
+        pa_destroy(eng);
+        if (st == PA_ERR_ARG) {
+            say("Unknown unit test unitTest%s\n", unit_test.c_str());
+            return 1;
+        }
+        say("%s\n", st == PA_OK ? report : (st == PA_ERR_CHECK ? report : pa_strerror(st)));
+        say("Completed %s Unit Test\n", unit_test.c_str());
+        return st == PA_OK ? 0 : 100;
     }
     pa_serve_config sc{};
     sc.slots = 3;

@@ -27,8 +27,11 @@ _lib = None
 _STATUS = {
     0: "ok", -1: "invalid argument", -2: "block size outside the supported family",
     -3: "HIP runtime error", -4: "device memory or HBM budget exhausted", -5: "timed out",
-    -6: "unknown seed or ticket", -7: "unsupported request",
+    -6: "unknown seed or ticket", -7: "unsupported request", -8: "self-test failed",
 }
+
+SELFTESTS = ("CalculateCorrectionFloat", "SetFirstElementToZero", "ElementWiseProduct", "BinInt2float",
+             "ToBinaryArray")
 
 
 class EngineError(RuntimeError):
@@ -93,12 +96,14 @@ def build(force=False):
 
 
 def load_library():
-    """Load the in-tree HIP library; raises if it has not been built."""
+    """Load the in-tree HIP library (or the one PA_HIP_LIBRARY names, a build of the same sources
+    with other options); raises if it has not been built."""
     global _lib
     if _lib is not None:
         return _lib
-    if not _LIB.exists():
-        raise RuntimeError(f"{_LIB} is missing: build it with privacyamplification_amd.build() "
+    path = pathlib.Path(os.environ.get("PA_HIP_LIBRARY", _LIB))
+    if not path.exists():
+        raise RuntimeError(f"{path} is missing: build it with privacyamplification_amd.build() "
                            "(there is no CPU fallback)")
     # One HIP runtime per process: torch ships its own libamdhip64 (same soname).  Loading torch
     # first makes this library bind to that runtime, so device pointers and stream handles from
@@ -107,7 +112,7 @@ def load_library():
         import torch  # noqa: F401
     except ImportError:
         pass
-    L = ctypes.CDLL(str(_LIB))
+    L = ctypes.CDLL(str(path))
     u32p, u8p, vp = ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint8), ctypes.c_void_p
     eng = ctypes.c_void_p
     L.pa_config_default.argtypes = [ctypes.POINTER(PaConfig)]
@@ -151,6 +156,7 @@ def load_library():
                                    ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint64),
                                    ctypes.POINTER(ctypes.c_uint64)]
     L.pa_synth_bits.argtypes = [ctypes.c_uint64, ctypes.c_double, ctypes.c_uint64, u32p]
+    L.pa_selftest.argtypes = [eng, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint32]
     L.pa_strerror.argtypes = [ctypes.c_int]
     L.pa_strerror.restype = ctypes.c_char_p
     L.pa_version.restype = ctypes.c_char_p
@@ -160,7 +166,7 @@ def load_library():
                  "pa_batch_device", "pa_run_batch", "pa_serve_start", "pa_serve_submit",
                  "pa_serve_submit_many",
                  "pa_serve_wait", "pa_serve_stop", "pa_host_alloc", "pa_host_free",
-                 "pa_block_device_split", "pa_run_block_split", "pa_block_extents",
+                 "pa_block_device_split", "pa_run_block_split", "pa_block_extents", "pa_selftest",
                  "pa_block_device_noalloc"):
         getattr(L, name).restype = ctypes.c_int
     _lib = L
@@ -427,6 +433,14 @@ class Engine:
     def serve_stop(self):
         _check(self._L.pa_serve_stop(self._h), "pa_serve_stop")
         self._serve_keep = {}
+
+    def selftest(self, name):
+        """pa_selftest: (passed, report) for one of SELFTESTS (the reference's unitTest* modes)."""
+        buf = ctypes.create_string_buffer(512)
+        st = self._L.pa_selftest(self._h, name.encode(), buf, len(buf))
+        if st not in (0, -8):
+            _check(st, "pa_selftest")
+        return st == 0, buf.value.decode()
 
     def stage_timing(self, enable=True):
         _check(self._L.pa_stage_timing(self._h, 1 if enable else 0), "pa_stage_timing")
