@@ -220,6 +220,7 @@ def run_single(ctx, n, m, mode, steps, warmup, man, with_kernels=True):
         for i in range(steps):
             step(i)
         ib.record(stream)
+        ib.synchronize()
         st = eng.stage_times()
         eng.stage_timing(False)
         res["instrumented_ms_per_step"] = round(ia.elapsed_time(ib) / steps, 4)

@@ -164,6 +164,14 @@ Extents block_extents(const Geom& g) {
     x.alloc[PA_BUF_SPEC_SEED] = (N + 1) * 8;
     x.alloc[PA_BUF_BITS_KEY] = x.alloc[PA_BUF_BITS_SEED] = (uint64_t)(N / g.D) * 4;
     x.alloc[PA_BUF_NIB] = pa::nib_bytes(g.N, g.W);
+    if (g.lg1 == (uint32_t)pa::RT_LG1) {
+        // pass B writes tile ((t2 / RT_COLS) N1/2 + p) 2 RT_COLS + s RT_COLS + t2 mod RT_COLS for
+        // t2 < N2, p < N1/2, s < 2 (the last: N - 1); pass A' reads the same range
+        const uint64_t last = ((uint64_t)(g.N2 - 1) / pa::RT_COLS * (g.N1 / 2) + (g.N1 / 2 - 1)) * (2 * pa::RT_COLS) +
+                              pa::RT_COLS + (g.N2 - 1) % pa::RT_COLS;
+        x.touched[PA_BUF_TILES] = (last + 1) * 8;
+        x.alloc[PA_BUF_TILES] = N * 8;
+    }
     return x;
 }
 
@@ -178,6 +186,7 @@ struct Workspace {
     uint32_t L = 0;
     float2* Yk = nullptr;      // N complex (key rows / R')
     float2* Ys = nullptr;      // N+1 complex (seed rows / split seed spectrum)
+    float2* Rt = nullptr;      // N complex: pass B's output tiles (lg1 == RT_LG1 only)
     uint32_t* tbk = nullptr;   // N/D words
     uint32_t* tbs = nullptr;
     uint8_t* nib = nullptr;    // L/4 bytes
@@ -417,6 +426,7 @@ uint64_t workspace_bytes(const Geom& g) {
     if (g.N1 > 1) {
         b += N * sizeof(float2) + (N + 1) * sizeof(float2);
         b += 2 * (N / g.D) * 4 + pa::nib_bytes(g.N, g.W) + 192;
+        if (g.lg1 == (uint32_t)pa::RT_LG1) b += N * sizeof(float2);
     }
     b += 256 + ws_key_bytes(g.L) * 2 + ws_out_bytes(g.L);  // key/seed and output staging
     return b;
@@ -470,6 +480,7 @@ int get_workspace(pa_engine* e, const Geom& g, Workspace** out) {
         allocs.push_back({(void**)&w->tbk, (size_t)(g.N / g.D) * 4});
         allocs.push_back({(void**)&w->tbs, (size_t)(g.N / g.D) * 4});
         allocs.push_back({(void**)&w->nib, (size_t)pa::nib_bytes(g.N, g.W)});
+        if (g.lg1 == (uint32_t)pa::RT_LG1) allocs.push_back({(void**)&w->Rt, (size_t)g.N * sizeof(float2)});
     }
     for (auto& a : allocs) {
         rc = dev_alloc(e, a.p, a.b);
@@ -587,22 +598,23 @@ int run_device_body(pa_engine* e, const Geom& g, Workspace* w, const uint32_t* k
         PA_HIP(hipStreamWaitEvent(s, su->done, 0));
         PA_HIP(timed(e, ST_TRANSPOSE_SEED, s, [&] { return pa::launch_transpose(su->dev, w->tbs, seed_cnt, g, 1, s); }));
         PA_HIP(timed(e, ST_FWD_COLS_SEED, s, [&] { return pa::launch_fwd_cols(w->tbs, w->Ys, g, t, s); }));
-        PA_HIP(timed(e, ST_ROWS, s, [&] { return pa::launch_rows(pa::ROWS_DYN, w->Yk, w->Ys, g, t, e->cus, s); }));
+        PA_HIP(timed(e, ST_ROWS, s, [&] { return pa::launch_rows(pa::ROWS_DYN, w->Yk, w->Ys, w->Rt, g, t, e->cus, s); }));
         ws_dev = seed_cnt;
     } else if (!sspec) {  // key and seed share each launch of the bit transpose and of pass A
         PA_HIP(timed(e, ST_TRANSPOSE_BOTH, s, [&] {
             return pa::launch_transpose(key, w->tbk, key_cnt, g, 0, s, seed, w->tbs);
         }));
         PA_HIP(timed(e, ST_FWD_COLS_BOTH, s, [&] { return pa::launch_fwd_cols(w->tbk, w->Yk, g, t, s, w->tbs, w->Ys); }));
-        PA_HIP(timed(e, ST_ROWS, s, [&] { return pa::launch_rows(pa::ROWS_DYN, w->Yk, w->Ys, g, t, e->cus, s); }));
+        PA_HIP(timed(e, ST_ROWS, s, [&] { return pa::launch_rows(pa::ROWS_DYN, w->Yk, w->Ys, w->Rt, g, t, e->cus, s); }));
         ws_dev = seed_cnt;
     } else {
         float2* S = const_cast<float2*>(sspec);
         PA_HIP(timed(e, ST_TRANSPOSE_KEY, s, [&] { return pa::launch_transpose(key, w->tbk, key_cnt, g, 0, s); }));
         PA_HIP(timed(e, ST_FWD_COLS_KEY, s, [&] { return pa::launch_fwd_cols(w->tbk, w->Yk, g, t, s); }));
-        PA_HIP(timed(e, ST_ROWS, s, [&] { return pa::launch_rows(pa::ROWS_STATIC, w->Yk, S, g, t, e->cus, s); }));
+        PA_HIP(timed(e, ST_ROWS, s, [&] { return pa::launch_rows(pa::ROWS_STATIC, w->Yk, S, w->Rt, g, t, e->cus, s); }));
     }
-    PA_HIP(timed(e, ST_INV_COLS, s, [&] { return pa::launch_inv_cols(w->Yk, w->nib, g, key_cnt, ws_dev, t, margin, s); }));
+    const float2* R = g.lg1 == (uint32_t)pa::RT_LG1 ? w->Rt : w->Yk;
+    PA_HIP(timed(e, ST_INV_COLS, s, [&] { return pa::launch_inv_cols(R, w->nib, g, key_cnt, ws_dev, t, margin, s); }));
     PA_HIP(timed(e, ST_PACK, s, [&] { return pa::launch_pack(w->nib, key, out, g, s, w->counts); }));
     w->counts_zero = true;  // the pack stage clears the partials for the next block on the stream
     return PA_OK;
@@ -689,7 +701,7 @@ int pa_destroy(pa_engine* e) {
     }
     for (auto& kv : e->ws) {
         Workspace* w = kv.second.get();
-        void* ps[] = {w->Yk, w->Ys, w->tbk, w->tbs, w->nib, w->counts, w->d_key, w->d_seed, w->d_out};
+        void* ps[] = {w->Yk, w->Ys, w->Rt, w->tbk, w->tbs, w->nib, w->counts, w->d_key, w->d_seed, w->d_out};
         for (void* p : ps) (void)hipFree(p);
         if (w->last_done) (void)hipEventDestroy(w->last_done);
     }
@@ -764,7 +776,7 @@ int pa_seed_put(pa_engine* e, const uint32_t* seed, uint32_t n, uint32_t m, int3
         PA_HIP(pa::launch_zero_u32(sd.ws, pa::POP_SLOTS, s));
         PA_HIP(pa::launch_transpose(w->d_seed, w->tbs, sd.ws, g, 1, s));
         PA_HIP(pa::launch_fwd_cols(w->tbs, sd.spec, g, t, s));
-        PA_HIP(pa::launch_rows(pa::ROWS_SEEDPREP, nullptr, sd.spec, g, t, e->cus, s));
+        PA_HIP(pa::launch_rows(pa::ROWS_SEEDPREP, nullptr, sd.spec, nullptr, g, t, e->cus, s));
     }
     PA_HIP(hipStreamSynchronize(s));
     *id = e->next_seed++;
@@ -850,6 +862,7 @@ int pa_block_device_noalloc(pa_engine* e, const uint32_t* d_key, const uint32_t*
         w.tbs = reinterpret_cast<uint32_t*>(take((uint64_t)(g.N / g.D) * 4));
         w.nib = take(pa::nib_bytes(g.N, g.W));
         w.counts = reinterpret_cast<uint32_t*>(take(2 * pa::POP_SLOTS * 4));
+        if (g.lg1 == (uint32_t)pa::RT_LG1) w.Rt = reinterpret_cast<float2*>(take((uint64_t)g.N * 8));
         if (off > NOALLOC_SCRATCH_BYTES) return PA_ERR_UNSUPPORTED;
     }
     const bool timing = e->timing;  // stage timing creates events: not on this path
@@ -1017,6 +1030,7 @@ uint64_t batch_scratch_bytes(const Geom& g, bool dyn) {
     uint64_t b = align256((uint64_t)g.N * 8) + align256((uint64_t)(g.N / g.D) * 4);  // Yk, tbk
     if (dyn) b += align256((uint64_t)(g.N + 1) * 8) + align256((uint64_t)(g.N / g.D) * 4);  // Ys, tbs
     b += align256(pa::nib_bytes(g.N, g.W)) + align256(2 * pa::POP_SLOTS * 4);  // nib, counts
+    if (g.lg1 == (uint32_t)pa::RT_LG1) b += align256((uint64_t)g.N * 8);     // R' tiles
     return b;
 }
 
@@ -1232,6 +1246,7 @@ int batch_execute(pa_engine* e, BatchSlot& sl, BatchPlan& p, const pa_block* dev
                 }
                 a.nib = take(pa::nib_bytes(g.N, g.W));
                 a.counts = reinterpret_cast<uint32_t*>(take(2 * pa::POP_SLOTS * 4));
+                if (g.lg1 == (uint32_t)pa::RT_LG1) a.Rt = reinterpret_cast<float2*>(take((uint64_t)g.N * 8));
             }
             sl.htab[gr.tab0 + j] = a;
         }
@@ -2074,7 +2089,7 @@ int st_seed_spectrum(pa_engine* e, const uint32_t* h_seed, uint32_t n, std::vect
     PA_HIP(pa::launch_zero_u32(dws.as<uint32_t>(), pa::POP_SLOTS, s));
     PA_HIP(pa::launch_transpose(dseed.as<uint32_t>(), w->tbs, dws.as<uint32_t>(), g, 1, s));
     PA_HIP(pa::launch_fwd_cols(w->tbs, dS.as<float2>(), g, t, s));
-    PA_HIP(pa::launch_rows(pa::ROWS_SEEDPREP, nullptr, dS.as<float2>(), g, t, e->cus, s));
+    PA_HIP(pa::launch_rows(pa::ROWS_SEEDPREP, nullptr, dS.as<float2>(), nullptr, g, t, e->cus, s));
     S.resize(g.N + 1);
     std::vector<uint32_t> part(pa::POP_SLOTS);
     PA_HIP(hipMemcpyAsync(S.data(), dS.p, (size_t)(g.N + 1) * 8, hipMemcpyDeviceToHost, s));

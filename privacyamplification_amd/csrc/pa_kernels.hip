@@ -538,7 +538,7 @@ __device__ __forceinline__ RowMap row_map(uint32_t w, uint32_t N1, int tid) {
 
 template <int MODE, bool BATCH>
 __global__ __launch_bounds__(ROWS_NT) void k_rows(float2* __restrict__ Yk, float2* __restrict__ Sst,
-                                                  Geom g, const float2* __restrict__ tw14,
+                                                  float2* __restrict__ Rt, Geom g, const float2* __restrict__ tw14,
                                                   const float2* __restrict__ twlo,
                                                   const float2* __restrict__ twhi,
                                                   const BlockArgs* __restrict__ bt) {
@@ -547,6 +547,7 @@ __global__ __launch_bounds__(ROWS_NT) void k_rows(float2* __restrict__ Yk, float
         g = a.g;
         Yk = a.Yk;
         Sst = (MODE == ROWS_STATIC) ? const_cast<float2*>(a.sspec) : a.Ys;
+        Rt = a.Rt;
     }
     constexpr int N2 = 1 << ROW_LOG, NT = ROWS_NT, SEQS = seq_stride(N2);
     static_assert(N2 == 8192 && NT == 512, "row pass is written for 8192-point rows, 512 threads");
@@ -719,14 +720,29 @@ __global__ __launch_bounds__(ROWS_NT) void k_rows(float2* __restrict__ Yk, float
                 wrAn = tw_two_level(twlo, twhi, mn.rowA);
                 wrBn = tw_two_level(twlo, twhi, mn.rowB);
             }
-            float2* op = Yk + (size_t)row * N2 + s1j;
             float2 tw = base;
+            if (N1 == (1u << RT_LG1)) {
+                // tiles: pair w, side s1seq, t2 = s1j + 256 r -> tile row t2 / RT_COLS = s1j / RT_COLS
+                // + (256 / RT_COLS) r, one scalar offset per r
+                const __amdgpu_buffer_rsrc_t ro = make_rsrc(Rt, N1 * N2 * 8u);
+                const uint32_t vo = ((((uint32_t)s1j / RT_COLS) * (N1 >> 1) + w) * (2u * RT_COLS) +
+                                     (uint32_t)s1seq * RT_COLS + ((uint32_t)s1j % RT_COLS)) * 8u;
+                const uint32_t rstep = (256u / RT_COLS) * (N1 >> 1) * (2u * RT_COLS) * 8u;  // t2 -> t2 + 256
 #pragma unroll
-            for (int r = 0; r < 32; ++r) {
-                if ((r & 7) == 0 && r)  // re-anchor every 8 steps to bound the recurrence error
-                    tw = cmul(base, anchor[r / 8 - 1]);
-                st_nt(&op[256 * r], cmul(F[r], tw));
-                tw = cmul(tw, step);
+                for (int r = 0; r < 32; ++r) {
+                    if ((r & 7) == 0 && r) tw = cmul(base, anchor[r / 8 - 1]);
+                    buf_store_nt(cmul(F[r], tw), ro, vo, (uint32_t)r * rstep);
+                    tw = cmul(tw, step);
+                }
+            } else {
+                float2* op = Yk + (size_t)row * N2 + s1j;
+#pragma unroll
+                for (int r = 0; r < 32; ++r) {
+                    if ((r & 7) == 0 && r)  // re-anchor every 8 steps to bound the recurrence error
+                        tw = cmul(base, anchor[r / 8 - 1]);
+                    st_nt(&op[256 * r], cmul(F[r], tw));
+                    tw = cmul(tw, step);
+                }
             }
         }
         __syncthreads();  // stage-3 reads done before the next item writes LDS
@@ -743,7 +759,8 @@ __global__ __launch_bounds__(ROWS_NT) void k_rows(float2* __restrict__ Yk, float
 
 // ------------------------------------------------------------------ pass A' ----
 // Workgroup grp owns columns [W*grp, W*grp + W): G[t1] = FFT_N1(R'[.][t2])[t1] = conj of the
-// unnormalised inverse, so c[2t] = Re G / N and c[2t+1] = -Im G / N at t = t2 + N2*t1.
+// unnormalised inverse, so c[2t] = Re G / N and c[2t+1] = -Im G / N at t = t2 + N2*t1.  R' is
+// row-major, or pass B's tiles when N1 = 2^RT_LG1.
 // Writes one nibble (4 conv bits, MSB first) per (column pair, t1) inside the output window, in
 // column-pair-major order nibT[pair * N1 + t1] so that each workgroup's writes are contiguous.
 template <int LG1, bool BATCH>
@@ -755,7 +772,7 @@ __global__ __launch_bounds__(inv_cols_threads(LG1)) void k_inv_cols(
     if constexpr (BATCH) {
         const BlockArgs& a = bt[blockIdx.z];
         g = a.g;
-        Rp = a.Yk;
+        Rp = LG1 == RT_LG1 ? a.Rt : a.Yk;
         nibT = a.nib;
         wk_cnt = a.counts;
         ws_cnt = a.seed ? a.counts + POP_SLOTS : a.ws;
@@ -766,7 +783,7 @@ __global__ __launch_bounds__(inv_cols_threads(LG1)) void k_inv_cols(
     constexpr int NT = inv_cols_threads(LG1);
     constexpr int LGL = LG1 < 14 ? LG1 : 14;  // LDS transform length (N1 = 2^15: two halves)
     constexpr int SEQS = seq_stride(1 << LGL);
-    constexpr int WL = LG1 == 13 ? 2 : W;      // columns held in LDS at once
+    constexpr int WL = LG1 == RT_LG1 ? 2 : W;  // columns held in LDS at once
     __shared__ float2 buf[WL * SEQS];
     __shared__ float2 twl[TWL_SIZE];
     load_twl(twl, tw14, NT);
@@ -817,49 +834,86 @@ __global__ __launch_bounds__(inv_cols_threads(LG1)) void k_inv_cols(
         if (margin) margin_publish(dist, margin);
         return;
     } else {
-    if constexpr (LG1 == 13) {
-        // Four columns t2 = 4 grp + 2 p + col as two pairs p through a two-column buffer.  Plan
-        // 32 x 16 x 16 per pair: stage 1 straight from global memory (thread t: column t & 1,
-        // butterfly t >> 1, rows j + 256 r), stage 3 into registers (thread t: butterfly t of both
-        // columns, t1 = t + 512 r), then the epilogue.  Both pairs' loads are issued first, so the
-        // second pair arrives while the first is transformed, and each 128-byte row segment serves
-        // 32 bytes to this workgroup.
+    if constexpr (LG1 == RT_LG1) {
+        // Four columns t2 = 4 grp + c from pass B's tiles: thread (p = t & 1, j = t >> 1) reads
+        // columns 2p, 2p + 1 of rows k1 = j + 256 r with one 16-byte load per row; rows k1 < N1/2
+        // are side 0 of pair k1, the others side 1 of pair (N1 - k1) mod N1/2.  The columns go
+        // through a two-column LDS buffer as {0, 2}, then {1, 3}; plan 32 x 16 x 16 (stage 1 from
+        // the loaded registers, stage 3 into registers, thread t: butterfly t of both columns,
+        // t1 = t + 512 r).  The first round's bits wait packed in registers (2 per t1 and
+        // column) for their partner columns 1 and 3.
         static_assert(W == 4 && NT == 512, "fused column pass expects 4 columns, 512 threads");
+        static_assert(RT_COLS % W == 0, "a workgroup's columns lie in one tile");
         __shared__ float2 ttf[32 * 17];
         fill_stage_table<16, 32>(ttf, tw14, NT);
-        const int col = threadIdx.x & 1, j = threadIdx.x >> 1;
-        float2 va[32], vb[32];
+        const int p = threadIdx.x & 1, j = threadIdx.x >> 1;
+        float2 v0[32], v1[32];
         {
-            const float2* rp = Rp + (size_t)j * g.N2 + grp * W + col;
+            const uint32_t c0 = grp * W;  // first column
+            const float2* tp = Rp + (size_t)(c0 / RT_COLS) * (N1 / 2) * (2 * RT_COLS) + (c0 % RT_COLS) + 2 * p;
 #pragma unroll
-            for (int r = 0; r < 32; ++r) va[r] = rp[(size_t)256 * r * g.N2];
-#pragma unroll
-            for (int r = 0; r < 32; ++r) vb[r] = rp[(size_t)256 * r * g.N2 + 2];
+            for (int r = 0; r < 32; ++r) {
+                const uint32_t k1 = (uint32_t)j + 256u * r;
+                const uint32_t pr = r < 16 ? k1 : ((N1 - k1) & (N1 / 2 - 1));
+                const float4 x = *reinterpret_cast<const float4*>(tp + (size_t)pr * (2 * RT_COLS) + (r < 16 ? 0 : RT_COLS));
+                v0[r] = make_float2(x.x, x.y);
+                v1[r] = make_float2(x.z, x.w);
+            }
         }
         const float corr = correction(pop_total(wk_cnt), pop_total(ws_cnt), g.L);
         const float invN = g.inv_N;
         float dist = 0.f;
+        const uint32_t t2a = grp * W, t2b = grp * W + 2u;  // even columns of the two pairs
+        auto inwin = [&](uint32_t t2, int r) {
+            const uint32_t i0 = 2u * (t2 + g.N2 * (threadIdx.x + 512u * r));
+            return !(i0 + 4u <= g.out_off || i0 >= g.out_off + g.out_bits);
+        };
         __syncthreads();  // twl, ttf
-#pragma unroll
-        for (int p = 0; p < 2; ++p) {
-            if (p == 1) __syncthreads();  // the first pair's stage-3 reads are done
-            stage_first_to_lds<32>(p == 0 ? va : vb, buf + col * SEQS, j);
-            __syncthreads();
-            stockham_stage<16, 32, LG1, 2, SEQS, NT, true>(buf, ttf);
+        uint32_t lo_a = 0, lo_b = 0;
+        // round 1: columns 0 and 2 (the even column of each pair)
+        stage_first_to_lds<32>(v0, buf + p * SEQS, j);
+        __syncthreads();
+        stockham_stage<16, 32, LG1, 2, SEQS, NT, true>(buf, ttf);
+        {
             float2 G0[16], G1[16];
             stage_last_from_lds<16, N1>(G0, buf, threadIdx.x, twl);
             stage_last_from_lds<16, N1>(G1, buf + SEQS, threadIdx.x, twl);
-            const uint32_t t2 = grp * W + 2u * (uint32_t)p;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                if (inwin(t2a, r)) {
+                    const uint32_t b0 = round_bit(G0[r].x * invN + corr, dist);
+                    const uint32_t b1 = round_bit(-G0[r].y * invN + corr, dist);
+                    lo_a |= ((b0 << 1) | b1) << (2 * r);
+                }
+                if (inwin(t2b, r)) {
+                    const uint32_t b0 = round_bit(G1[r].x * invN + corr, dist);
+                    const uint32_t b1 = round_bit(-G1[r].y * invN + corr, dist);
+                    lo_b |= ((b0 << 1) | b1) << (2 * r);
+                }
+            }
+        }
+        __syncthreads();  // round-1 stage-3 reads are done
+        // round 2: columns 1 and 3, joined with their partners' bits
+        stage_first_to_lds<32>(v1, buf + p * SEQS, j);
+        __syncthreads();
+        stockham_stage<16, 32, LG1, 2, SEQS, NT, true>(buf, ttf);
+        {
+            float2 G0[16], G1[16];
+            stage_last_from_lds<16, N1>(G0, buf, threadIdx.x, twl);
+            stage_last_from_lds<16, N1>(G1, buf + SEQS, threadIdx.x, twl);
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const uint32_t t1 = threadIdx.x + 512u * r;
-                const uint32_t i0 = 2u * (t2 + g.N2 * t1);
-                if (i0 + 4u <= g.out_off || i0 >= g.out_off + g.out_bits) continue;
-                const uint32_t b0 = round_bit(G0[r].x * invN + corr, dist);
-                const uint32_t b1 = round_bit(-G0[r].y * invN + corr, dist);
-                const uint32_t b2 = round_bit(G1[r].x * invN + corr, dist);
-                const uint32_t b3 = round_bit(-G1[r].y * invN + corr, dist);
-                st_nt_u8(&nibT[(size_t)(t2 >> 1) * N1 + t1], (uint8_t)((b0 << 3) | (b1 << 2) | (b2 << 1) | b3));
+                if (inwin(t2a, r)) {
+                    const uint32_t b2 = round_bit(G0[r].x * invN + corr, dist);
+                    const uint32_t b3 = round_bit(-G0[r].y * invN + corr, dist);
+                    st_nt_u8(&nibT[(size_t)(t2a >> 1) * N1 + t1], (uint8_t)((((lo_a >> (2 * r)) & 3u) << 2) | (b2 << 1) | b3));
+                }
+                if (inwin(t2b, r)) {
+                    const uint32_t b2 = round_bit(G1[r].x * invN + corr, dist);
+                    const uint32_t b3 = round_bit(-G1[r].y * invN + corr, dist);
+                    st_nt_u8(&nibT[(size_t)(t2b >> 1) * N1 + t1], (uint8_t)((((lo_b >> (2 * r)) & 3u) << 2) | (b2 << 1) | b3));
+                }
             }
         }
         if (margin) margin_publish(dist, margin);
@@ -1275,13 +1329,14 @@ hipError_t launch_fwd_cols(const uint32_t* tb, float2* Y, const Geom& g, const T
     }
 }
 
-hipError_t launch_rows(int mode, float2* Yk, float2* S, const Geom& g, const Tables& t, int cus, hipStream_t s) {
+hipError_t launch_rows(int mode, float2* Yk, float2* S, float2* Rt, const Geom& g, const Tables& t, int cus,
+                       hipStream_t s) {
     const uint32_t items = g.N1 / 2u;
     const uint32_t grid = items < (uint32_t)cus ? items : (uint32_t)cus;  // one workgroup per CU
     switch (mode) {
-    case ROWS_STATIC: k_rows<ROWS_STATIC, false><<<grid, ROWS_NT, 0, s>>>(Yk, S, g, t.tw14, t.lo, t.hi, nullptr); break;
-    case ROWS_DYN: k_rows<ROWS_DYN, false><<<grid, ROWS_NT, 0, s>>>(Yk, S, g, t.tw14, t.lo, t.hi, nullptr); break;
-    case ROWS_SEEDPREP: k_rows<ROWS_SEEDPREP, false><<<grid, ROWS_NT, 0, s>>>(Yk, S, g, t.tw14, t.lo, t.hi, nullptr); break;
+    case ROWS_STATIC: k_rows<ROWS_STATIC, false><<<grid, ROWS_NT, 0, s>>>(Yk, S, Rt, g, t.tw14, t.lo, t.hi, nullptr); break;
+    case ROWS_DYN: k_rows<ROWS_DYN, false><<<grid, ROWS_NT, 0, s>>>(Yk, S, Rt, g, t.tw14, t.lo, t.hi, nullptr); break;
+    case ROWS_SEEDPREP: k_rows<ROWS_SEEDPREP, false><<<grid, ROWS_NT, 0, s>>>(Yk, S, Rt, g, t.tw14, t.lo, t.hi, nullptr); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1426,8 +1481,8 @@ hipError_t launch_batch_rows(int mode, const BlockArgs* bt, uint32_t count, cons
     if (grid_x == 0 || grid_x > items) grid_x = items;
     const dim3 grid(grid_x, 1, count);
     switch (mode) {
-    case ROWS_STATIC: k_rows<ROWS_STATIC, true><<<grid, ROWS_NT, 0, s>>>(nullptr, nullptr, g0, t.tw14, t.lo, t.hi, bt); break;
-    case ROWS_DYN: k_rows<ROWS_DYN, true><<<grid, ROWS_NT, 0, s>>>(nullptr, nullptr, g0, t.tw14, t.lo, t.hi, bt); break;
+    case ROWS_STATIC: k_rows<ROWS_STATIC, true><<<grid, ROWS_NT, 0, s>>>(nullptr, nullptr, nullptr, g0, t.tw14, t.lo, t.hi, bt); break;
+    case ROWS_DYN: k_rows<ROWS_DYN, true><<<grid, ROWS_NT, 0, s>>>(nullptr, nullptr, nullptr, g0, t.tw14, t.lo, t.hi, bt); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

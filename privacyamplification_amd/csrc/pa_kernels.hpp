@@ -68,10 +68,13 @@ hipError_t launch_transpose(const uint32_t* bits, uint32_t* tb, uint32_t* count,
 hipError_t launch_fwd_cols(const uint32_t* tb, float2* Y, const Geom& g, const Tables& t, hipStream_t s,
                            const uint32_t* tb2 = nullptr, float2* Y2 = nullptr);
 // ROWS_SEEDPREP: S <- split spectrum of the seed rows held in S (in place, N+1 values).
-// ROWS_STATIC:   Yk <- R' from the key rows in Yk and the split seed spectrum S.
-// ROWS_DYN:      Yk <- R' from the key rows in Yk and the seed rows (pass-A output) in S.
+// ROWS_STATIC:   R' from the key rows in Yk and the split seed spectrum S.
+// ROWS_DYN:      R' from the key rows in Yk and the seed rows (pass-A output) in S.
+// R' goes to Rt in tiles when g.lg1 == RT_LG1 (N complex values), else to Yk in place.
 // Persistent: at most `cus` workgroups (one per compute unit) walk the row pairs.
-hipError_t launch_rows(int mode, float2* Yk, float2* S, const Geom& g, const Tables& t, int cus, hipStream_t s);
+hipError_t launch_rows(int mode, float2* Yk, float2* S, float2* Rt, const Geom& g, const Tables& t, int cus,
+                       hipStream_t s);
+// R: pass B's output (the tile buffer when g.lg1 == RT_LG1).
 hipError_t launch_inv_cols(const float2* R, uint8_t* nib, const Geom& g, const uint32_t* wk,
                            const uint32_t* ws, const Tables& t, float* margin, hipStream_t s);
 // zero_counts != null: the launch also clears those 2 * POP_SLOTS popcount partials.

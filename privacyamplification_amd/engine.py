@@ -200,7 +200,7 @@ def out_bytes(n, m, do_compress=True):
     return m // 8 if do_compress else n // 16
 
 
-BUFFERS = ("key", "seed", "out", "spec_key", "spec_seed", "bits_key", "bits_seed", "nib", "counts")
+BUFFERS = ("key", "seed", "out", "spec_key", "spec_seed", "bits_key", "bits_seed", "nib", "counts", "tiles")
 
 
 def block_extents(n, m, flags=DEFAULT_FLAGS):
