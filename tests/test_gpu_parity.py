@@ -130,12 +130,15 @@ def test_non_power_of_two_blocks(engine, oracle, n, m):
 
 
 @pytest.mark.parametrize("n,m", [((1 << 23) + 32, 1 << 20), (8664960, 1083104), (10852704, 2713152),
-                                 (20229632, 5057408), (26088768, 6522176), ((1 << 25) - 32, 3 << 22)])
+                                 (20229632, 5057408), (26088768, 6522176), ((1 << 25) - 32, 3 << 22),
+                                 ((1 << 26) + 32, 1 << 24), (3 << 25, 3 << 23)])
 def test_large_non_power_of_two_blocks(engine, oracle, n, m):
     """Non-2^k blocks whose transform length L = 2^ceil(log2 n) leaves up to half of L beyond n:
     the engine repeats the rotated seed up to L, so the values before rounding stay small (a zero
     pad there makes them swing by about h/4 and FP32 rounding fails from L = 2^24 on).  Checked
-    against the exact NTT oracle, one block at a time and in one batch, with the rounding margin."""
+    against the exact NTT oracle, one block at a time and in one batch, with the rounding margin.
+    The last two run on the 2^27 transform, whose pass B writes tiles for pass A', with output
+    windows that start inside the transform."""
     import torch
     say(f"large non-2^k n={n} m={m}")
     key = oracle.synth_bits(0xA00 + n % 997, n + 1)
