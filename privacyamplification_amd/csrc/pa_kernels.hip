@@ -764,7 +764,7 @@ __global__ __launch_bounds__(ROWS_NT) void k_rows(float2* __restrict__ Yk, float
 // Writes one nibble (4 conv bits, MSB first) per (column pair, t1) inside the output window, in
 // column-pair-major order nibT[pair * N1 + t1] so that each workgroup's writes are contiguous.
 template <int LG1, bool BATCH>
-__global__ __launch_bounds__(inv_cols_threads(LG1)) void k_inv_cols(
+__global__ __launch_bounds__(inv_cols_threads(LG1), LG1 == RT_LG1 ? 2 : 1) void k_inv_cols(
     const float2* __restrict__ Rp, uint8_t* __restrict__ nibT, Geom g,
     const uint32_t* __restrict__ wk_cnt, const uint32_t* __restrict__ ws_cnt,
     const float2* __restrict__ tw14, const float2* __restrict__ twlo, const float2* __restrict__ twhi,
@@ -783,7 +783,7 @@ __global__ __launch_bounds__(inv_cols_threads(LG1)) void k_inv_cols(
     constexpr int NT = inv_cols_threads(LG1);
     constexpr int LGL = LG1 < 14 ? LG1 : 14;  // LDS transform length (N1 = 2^15: two halves)
     constexpr int SEQS = seq_stride(1 << LGL);
-    constexpr int WL = LG1 == RT_LG1 ? 2 : W;  // columns held in LDS at once
+    constexpr int WL = LG1 == RT_LG1 ? 1 : W;  // columns held in LDS at once
     __shared__ float2 buf[WL * SEQS];
     __shared__ float2 twl[TWL_SIZE];
     load_twl(twl, tw14, NT);
@@ -835,22 +835,23 @@ __global__ __launch_bounds__(inv_cols_threads(LG1)) void k_inv_cols(
         return;
     } else {
     if constexpr (LG1 == RT_LG1) {
-        // Four columns t2 = 4 grp + c from pass B's tiles: thread (p = t & 1, j = t >> 1) reads
-        // columns 2p, 2p + 1 of rows k1 = j + 256 r with one 16-byte load per row; rows k1 < N1/2
-        // are side 0 of pair k1, the others side 1 of pair (N1 - k1) mod N1/2.  The columns go
-        // through a two-column LDS buffer as {0, 2}, then {1, 3}; plan 32 x 16 x 16 (stage 1 from
-        // the loaded registers, stage 3 into registers, thread t: butterfly t of both columns,
-        // t1 = t + 512 r).  The first round's bits wait packed in registers (2 per t1 and
-        // column) for their partner columns 1 and 3.
-        static_assert(W == 4 && NT == 512, "fused column pass expects 4 columns, 512 threads");
+        // One column pair t2 = 2 grp, 2 grp + 1 from pass B's tiles per 256-thread workgroup, one
+        // column in LDS at a time (74 KB), so that two workgroups share a compute unit and one's
+        // reads overlap the other's transforms.  Thread j loads both columns of row k1 = j + 256 r
+        // with one 16-byte load: rows k1 < N1/2 are side 0 of pair k1, the others side 1 of pair
+        // (N1 - k1) mod N1/2, so a workgroup walks consecutive lines.  Plan 32 x 16 x 16 per
+        // column: stage 1 from the loaded registers, stage 2 in LDS, stage 3 into registers for
+        // butterflies b = j and j + 256 (t1 = b + 512 r).  The first column's bits wait packed in
+        // registers (2 per t1) for the second's.
+        static_assert(W == 2 && NT == 256, "tiled column pass expects 2 columns, 256 threads");
         static_assert(RT_COLS % W == 0, "a workgroup's columns lie in one tile");
         __shared__ float2 ttf[32 * 17];
         fill_stage_table<16, 32>(ttf, tw14, NT);
-        const int p = threadIdx.x & 1, j = threadIdx.x >> 1;
+        const int j = threadIdx.x;
         float2 v0[32], v1[32];
         {
-            const uint32_t c0 = grp * W;  // first column
-            const float2* tp = Rp + (size_t)(c0 / RT_COLS) * (N1 / 2) * (2 * RT_COLS) + (c0 % RT_COLS) + 2 * p;
+            const uint32_t c0 = grp * W;
+            const float2* tp = Rp + (size_t)(c0 / RT_COLS) * (N1 / 2) * (2 * RT_COLS) + (c0 % RT_COLS);
 #pragma unroll
             for (int r = 0; r < 32; ++r) {
                 const uint32_t k1 = (uint32_t)j + 256u * r;
@@ -863,57 +864,45 @@ __global__ __launch_bounds__(inv_cols_threads(LG1)) void k_inv_cols(
         const float corr = correction(pop_total(wk_cnt), pop_total(ws_cnt), g.L);
         const float invN = g.inv_N;
         float dist = 0.f;
-        const uint32_t t2a = grp * W, t2b = grp * W + 2u;  // even columns of the two pairs
-        auto inwin = [&](uint32_t t2, int r) {
-            const uint32_t i0 = 2u * (t2 + g.N2 * (threadIdx.x + 512u * r));
+        const uint32_t t2 = grp * W;
+        auto inwin = [&](uint32_t t1) {
+            const uint32_t i0 = 2u * (t2 + g.N2 * t1);
             return !(i0 + 4u <= g.out_off || i0 >= g.out_off + g.out_bits);
         };
         __syncthreads();  // twl, ttf
-        uint32_t lo_a = 0, lo_b = 0;
-        // round 1: columns 0 and 2 (the even column of each pair)
-        stage_first_to_lds<32>(v0, buf + p * SEQS, j);
+        stage_first_to_lds<32>(v0, buf, j);
         __syncthreads();
-        stockham_stage<16, 32, LG1, 2, SEQS, NT, true>(buf, ttf);
-        {
-            float2 G0[16], G1[16];
-            stage_last_from_lds<16, N1>(G0, buf, threadIdx.x, twl);
-            stage_last_from_lds<16, N1>(G1, buf + SEQS, threadIdx.x, twl);
+        stockham_stage<16, 32, LG1, 1, SEQS, NT, true>(buf, ttf);
+        uint32_t lo[2] = {0u, 0u};
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            float2 G[16];
+            const int bf = j + 256 * h;
+            stage_last_from_lds<16, N1>(G, buf, bf, twl);
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                if (inwin(t2a, r)) {
-                    const uint32_t b0 = round_bit(G0[r].x * invN + corr, dist);
-                    const uint32_t b1 = round_bit(-G0[r].y * invN + corr, dist);
-                    lo_a |= ((b0 << 1) | b1) << (2 * r);
-                }
-                if (inwin(t2b, r)) {
-                    const uint32_t b0 = round_bit(G1[r].x * invN + corr, dist);
-                    const uint32_t b1 = round_bit(-G1[r].y * invN + corr, dist);
-                    lo_b |= ((b0 << 1) | b1) << (2 * r);
-                }
+                if (!inwin((uint32_t)bf + 512u * r)) continue;
+                const uint32_t b0 = round_bit(G[r].x * invN + corr, dist);
+                const uint32_t b1 = round_bit(-G[r].y * invN + corr, dist);
+                lo[h] |= ((b0 << 1) | b1) << (2 * r);
             }
         }
-        __syncthreads();  // round-1 stage-3 reads are done
-        // round 2: columns 1 and 3, joined with their partners' bits
-        stage_first_to_lds<32>(v1, buf + p * SEQS, j);
+        __syncthreads();  // the first column's stage-3 reads are done
+        stage_first_to_lds<32>(v1, buf, j);
         __syncthreads();
-        stockham_stage<16, 32, LG1, 2, SEQS, NT, true>(buf, ttf);
-        {
-            float2 G0[16], G1[16];
-            stage_last_from_lds<16, N1>(G0, buf, threadIdx.x, twl);
-            stage_last_from_lds<16, N1>(G1, buf + SEQS, threadIdx.x, twl);
+        stockham_stage<16, 32, LG1, 1, SEQS, NT, true>(buf, ttf);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            float2 G[16];
+            const int bf = j + 256 * h;
+            stage_last_from_lds<16, N1>(G, buf, bf, twl);
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                const uint32_t t1 = threadIdx.x + 512u * r;
-                if (inwin(t2a, r)) {
-                    const uint32_t b2 = round_bit(G0[r].x * invN + corr, dist);
-                    const uint32_t b3 = round_bit(-G0[r].y * invN + corr, dist);
-                    st_nt_u8(&nibT[(size_t)(t2a >> 1) * N1 + t1], (uint8_t)((((lo_a >> (2 * r)) & 3u) << 2) | (b2 << 1) | b3));
-                }
-                if (inwin(t2b, r)) {
-                    const uint32_t b2 = round_bit(G1[r].x * invN + corr, dist);
-                    const uint32_t b3 = round_bit(-G1[r].y * invN + corr, dist);
-                    st_nt_u8(&nibT[(size_t)(t2b >> 1) * N1 + t1], (uint8_t)((((lo_b >> (2 * r)) & 3u) << 2) | (b2 << 1) | b3));
-                }
+                const uint32_t t1 = (uint32_t)bf + 512u * r;
+                if (!inwin(t1)) continue;
+                const uint32_t b2 = round_bit(G[r].x * invN + corr, dist);
+                const uint32_t b3 = round_bit(-G[r].y * invN + corr, dist);
+                st_nt_u8(&nibT[(size_t)grp * N1 + t1], (uint8_t)((((lo[h] >> (2 * r)) & 3u) << 2) | (b2 << 1) | b3));
             }
         }
         if (margin) margin_publish(dist, margin);

@@ -36,13 +36,13 @@ __host__ __device__ constexpr int cls_table_offset(int D) {
 constexpr int CLS_TABLE_TOTAL = cls_table_offset(32);  // D = 1 .. 16
 
 // Columns per workgroup in pass A' (tile of W x N1 complex <= 16384 values in LDS; N1 = 8192 takes
-// 4 columns as two pairs through a 2-column buffer) and its threads.
+// 2 columns one at a time, two workgroups per compute unit) and its threads.
 __host__ __device__ constexpr int inv_cols_width(int lg1) {
-    return lg1 >= 14 ? 1 : (lg1 == 13 ? 4 : ((14 - lg1) >= 4 ? 16 : (1 << (14 - lg1))));
+    return lg1 >= 14 ? 1 : (lg1 == 13 ? 2 : ((14 - lg1) >= 4 ? 16 : (1 << (14 - lg1))));
 }
 __host__ __device__ constexpr int clamp_threads(int t) { return t < 64 ? 64 : (t > 1024 ? 1024 : t); }
 __host__ __device__ constexpr int inv_cols_threads(int lg1) {
-    return lg1 == 13 ? 512 : (lg1 >= 14 ? 1024 : clamp_threads(inv_cols_width(lg1) * (1 << lg1) / 16));
+    return lg1 == 13 ? 256 : (lg1 >= 14 ? 1024 : clamp_threads(inv_cols_width(lg1) * (1 << lg1) / 16));
 }
 // Columns per workgroup in pass A (16; 8 when the class length M = 2048 would overflow LDS).
 __host__ __device__ constexpr int fwd_cols_width(int lgm) { return lgm >= 11 ? 8 : 16; }
