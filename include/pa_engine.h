@@ -235,7 +235,7 @@ enum pa_buffer {
     PA_BUF_BITS_SEED,      /* seed bit masks */
     PA_BUF_NIB,            /* rounded output bits before packing */
     PA_BUF_COUNTS,         /* popcount partials */
-    PA_BUF_TILES,          /* pass B's output in tiles for pass A' (2^27-bit transform length) */
+    PA_BUF_TILES,          /* pass B's output in tiles for pass A' (transform lengths 2^24..2^27) */
     PA_BUF_COUNT
 };
 /* Buffer extents of a block shape, a diagnostic with no counterpart in the reference: touched[b]

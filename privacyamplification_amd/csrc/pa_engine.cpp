@@ -164,7 +164,7 @@ Extents block_extents(const Geom& g) {
     x.alloc[PA_BUF_SPEC_SEED] = (N + 1) * 8;
     x.alloc[PA_BUF_BITS_KEY] = x.alloc[PA_BUF_BITS_SEED] = (uint64_t)(N / g.D) * 4;
     x.alloc[PA_BUF_NIB] = pa::nib_bytes(g.N, g.W);
-    if (g.lg1 == (uint32_t)pa::RT_LG1) {
+    if (pa::rt_tiled(g.lg1)) {
         // pass B writes tile ((t2 / RT_COLS) N1/2 + p) 2 RT_COLS + s RT_COLS + t2 mod RT_COLS for
         // t2 < N2, p < N1/2, s < 2 (the last: N - 1); pass A' reads the same range
         const uint64_t last = ((uint64_t)(g.N2 - 1) / pa::RT_COLS * (g.N1 / 2) + (g.N1 / 2 - 1)) * (2 * pa::RT_COLS) +
@@ -186,7 +186,7 @@ struct Workspace {
     uint32_t L = 0;
     float2* Yk = nullptr;      // N complex (key rows / R')
     float2* Ys = nullptr;      // N+1 complex (seed rows / split seed spectrum)
-    float2* Rt = nullptr;      // N complex: pass B's output tiles (lg1 == RT_LG1 only)
+    float2* Rt = nullptr;      // N complex: pass B's output tiles (rt_tiled(lg1) only)
     uint32_t* tbk = nullptr;   // N/D words
     uint32_t* tbs = nullptr;
     uint8_t* nib = nullptr;    // L/4 bytes
@@ -426,7 +426,7 @@ uint64_t workspace_bytes(const Geom& g) {
     if (g.N1 > 1) {
         b += N * sizeof(float2) + (N + 1) * sizeof(float2);
         b += 2 * (N / g.D) * 4 + pa::nib_bytes(g.N, g.W) + 192;
-        if (g.lg1 == (uint32_t)pa::RT_LG1) b += N * sizeof(float2);
+        if (pa::rt_tiled(g.lg1)) b += N * sizeof(float2);
     }
     b += 256 + ws_key_bytes(g.L) * 2 + ws_out_bytes(g.L);  // key/seed and output staging
     return b;
@@ -480,7 +480,7 @@ int get_workspace(pa_engine* e, const Geom& g, Workspace** out) {
         allocs.push_back({(void**)&w->tbk, (size_t)(g.N / g.D) * 4});
         allocs.push_back({(void**)&w->tbs, (size_t)(g.N / g.D) * 4});
         allocs.push_back({(void**)&w->nib, (size_t)pa::nib_bytes(g.N, g.W)});
-        if (g.lg1 == (uint32_t)pa::RT_LG1) allocs.push_back({(void**)&w->Rt, (size_t)g.N * sizeof(float2)});
+        if (pa::rt_tiled(g.lg1)) allocs.push_back({(void**)&w->Rt, (size_t)g.N * sizeof(float2)});
     }
     for (auto& a : allocs) {
         rc = dev_alloc(e, a.p, a.b);
@@ -613,7 +613,7 @@ int run_device_body(pa_engine* e, const Geom& g, Workspace* w, const uint32_t* k
         PA_HIP(timed(e, ST_FWD_COLS_KEY, s, [&] { return pa::launch_fwd_cols(w->tbk, w->Yk, g, t, s); }));
         PA_HIP(timed(e, ST_ROWS, s, [&] { return pa::launch_rows(pa::ROWS_STATIC, w->Yk, S, w->Rt, g, t, e->cus, s); }));
     }
-    const float2* R = g.lg1 == (uint32_t)pa::RT_LG1 ? w->Rt : w->Yk;
+    const float2* R = pa::rt_tiled(g.lg1) ? w->Rt : w->Yk;
     PA_HIP(timed(e, ST_INV_COLS, s, [&] { return pa::launch_inv_cols(R, w->nib, g, key_cnt, ws_dev, t, margin, s); }));
     PA_HIP(timed(e, ST_PACK, s, [&] { return pa::launch_pack(w->nib, key, out, g, s, w->counts); }));
     w->counts_zero = true;  // the pack stage clears the partials for the next block on the stream
@@ -862,7 +862,7 @@ int pa_block_device_noalloc(pa_engine* e, const uint32_t* d_key, const uint32_t*
         w.tbs = reinterpret_cast<uint32_t*>(take((uint64_t)(g.N / g.D) * 4));
         w.nib = take(pa::nib_bytes(g.N, g.W));
         w.counts = reinterpret_cast<uint32_t*>(take(2 * pa::POP_SLOTS * 4));
-        if (g.lg1 == (uint32_t)pa::RT_LG1) w.Rt = reinterpret_cast<float2*>(take((uint64_t)g.N * 8));
+        if (pa::rt_tiled(g.lg1)) w.Rt = reinterpret_cast<float2*>(take((uint64_t)g.N * 8));
         if (off > NOALLOC_SCRATCH_BYTES) return PA_ERR_UNSUPPORTED;
     }
     const bool timing = e->timing;  // stage timing creates events: not on this path
@@ -1030,7 +1030,7 @@ uint64_t batch_scratch_bytes(const Geom& g, bool dyn) {
     uint64_t b = align256((uint64_t)g.N * 8) + align256((uint64_t)(g.N / g.D) * 4);  // Yk, tbk
     if (dyn) b += align256((uint64_t)(g.N + 1) * 8) + align256((uint64_t)(g.N / g.D) * 4);  // Ys, tbs
     b += align256(pa::nib_bytes(g.N, g.W)) + align256(2 * pa::POP_SLOTS * 4);  // nib, counts
-    if (g.lg1 == (uint32_t)pa::RT_LG1) b += align256((uint64_t)g.N * 8);     // R' tiles
+    if (pa::rt_tiled(g.lg1)) b += align256((uint64_t)g.N * 8);     // R' tiles
     return b;
 }
 
@@ -1246,7 +1246,7 @@ int batch_execute(pa_engine* e, BatchSlot& sl, BatchPlan& p, const pa_block* dev
                 }
                 a.nib = take(pa::nib_bytes(g.N, g.W));
                 a.counts = reinterpret_cast<uint32_t*>(take(2 * pa::POP_SLOTS * 4));
-                if (g.lg1 == (uint32_t)pa::RT_LG1) a.Rt = reinterpret_cast<float2*>(take((uint64_t)g.N * 8));
+                if (pa::rt_tiled(g.lg1)) a.Rt = reinterpret_cast<float2*>(take((uint64_t)g.N * 8));
             }
             sl.htab[gr.tab0 + j] = a;
         }

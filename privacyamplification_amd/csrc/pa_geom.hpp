@@ -24,13 +24,18 @@ constexpr int ROW_LOG = 13;            // N2 = 8192 for the four-step path
 constexpr int SMALL_MAX_LOG_N = 13;    // N <= 8192 -> single-workgroup path
 constexpr int MAX_L_LOG = 29;          // largest transform served (L = 2^29; the reference stops at 2^27)
 
-// Pass B's output R'[k1][t2] when N1 = 2^RT_LG1 (the 2^27-bit transform) goes to its own buffer in
-// tiles: row pair p (rows p and N1-p, or 0 and N1/2 for p = 0), side s (0: row p, 1: its partner)
-// at Rt[((t2 / RT_COLS) N1/2 + p) 2 RT_COLS + s RT_COLS + t2 mod RT_COLS].  A tile holds both rows'
-// RT_COLS columns (2 x 128 bytes): pass B writes whole 128-byte lines, and a 4-column workgroup of
-// pass A' reads 32 bytes of consecutive lines along p instead of 32 bytes of rows 64 KiB apart.
+// Pass B's output R'[k1][t2] when 2^RT_LG1_MIN <= N1 <= 2^RT_LG1 (transforms of 2^24 to 2^27 bits)
+// goes to its own buffer in tiles: row pair p (rows p and N1-p, or 0 and N1/2 for p = 0), side s
+// (0: row p, 1: its partner) at Rt[((t2 / RT_COLS) N1/2 + p) 2 RT_COLS + s RT_COLS + t2 mod RT_COLS].
+// A tile holds both rows' RT_COLS columns (2 x 128 bytes): pass B writes whole 128-byte lines, and a
+// workgroup of pass A' (2 to 8 columns) reads its columns from consecutive lines along p instead of
+// from rows 8 N2 bytes apart.
+constexpr int RT_LG1_MIN = 10;
 constexpr int RT_LG1 = 13;
 constexpr int RT_COLS = 16;
+__host__ __device__ constexpr bool rt_tiled(uint32_t lg1) {
+    return lg1 >= (uint32_t)RT_LG1_MIN && lg1 <= (uint32_t)RT_LG1;
+}
 
 struct Geom {
     uint32_t n, m, h, kbits;           // block shape; kbits = h + 1
@@ -57,7 +62,7 @@ struct BlockArgs {
     float2* Ys;             // N+1 values (dynamic seed, four-step path only)
     uint32_t* tbk;
     uint32_t* tbs;
-    float2* Rt;             // R' tiles (N1 = 2^RT_LG1), else nullptr
+    float2* Rt;             // R' tiles (rt_tiled(lg1)), else nullptr
     uint8_t* nib;
     uint32_t* counts;       // key partial popcounts, then seed partial popcounts
     uint32_t* out;

@@ -721,7 +721,7 @@ __global__ __launch_bounds__(ROWS_NT) void k_rows(float2* __restrict__ Yk, float
                 wrBn = tw_two_level(twlo, twhi, mn.rowB);
             }
             float2 tw = base;
-            if (N1 == (1u << RT_LG1)) {
+            if (rt_tiled(g.lg1)) {
                 // tiles: pair w, side s1seq, t2 = s1j + 256 r -> tile row t2 / RT_COLS = s1j / RT_COLS
                 // + (256 / RT_COLS) r, one scalar offset per r
                 const __amdgpu_buffer_rsrc_t ro = make_rsrc(Rt, N1 * N2 * 8u);
@@ -760,11 +760,11 @@ __global__ __launch_bounds__(ROWS_NT) void k_rows(float2* __restrict__ Yk, float
 // ------------------------------------------------------------------ pass A' ----
 // Workgroup grp owns columns [W*grp, W*grp + W): G[t1] = FFT_N1(R'[.][t2])[t1] = conj of the
 // unnormalised inverse, so c[2t] = Re G / N and c[2t+1] = -Im G / N at t = t2 + N2*t1.  R' is
-// row-major, or pass B's tiles when N1 = 2^RT_LG1.
+// row-major, or pass B's tiles when rt_tiled(LG1).
 // Writes one nibble (4 conv bits, MSB first) per (column pair, t1) inside the output window, in
 // column-pair-major order nibT[pair * N1 + t1] so that each workgroup's writes are contiguous.
 template <int LG1, bool BATCH>
-__global__ __launch_bounds__(inv_cols_threads(LG1), LG1 == RT_LG1 ? 2 : 1) void k_inv_cols(
+__global__ __launch_bounds__(inv_cols_threads(LG1), rt_tiled(LG1) ? 2 : 1) void k_inv_cols(
     const float2* __restrict__ Rp, uint8_t* __restrict__ nibT, Geom g,
     const uint32_t* __restrict__ wk_cnt, const uint32_t* __restrict__ ws_cnt,
     const float2* __restrict__ tw14, const float2* __restrict__ twlo, const float2* __restrict__ twhi,
@@ -772,7 +772,7 @@ __global__ __launch_bounds__(inv_cols_threads(LG1), LG1 == RT_LG1 ? 2 : 1) void 
     if constexpr (BATCH) {
         const BlockArgs& a = bt[blockIdx.z];
         g = a.g;
-        Rp = LG1 == RT_LG1 ? a.Rt : a.Yk;
+        Rp = rt_tiled(LG1) ? a.Rt : a.Yk;
         nibT = a.nib;
         wk_cnt = a.counts;
         ws_cnt = a.seed ? a.counts + POP_SLOTS : a.ws;
@@ -913,7 +913,16 @@ __global__ __launch_bounds__(inv_cols_threads(LG1), LG1 == RT_LG1 ? 2 : 1) void 
 #pragma unroll
     for (int i = 0; i < LPT; ++i) {
         const int e = threadIdx.x + i * NT;
-        if (e < W * N1) v[i] = Rp[(size_t)(e / W) * g.N2 + grp * W + (e % W)];
+        if (e < W * N1) {
+            const uint32_t k1 = (uint32_t)(e / W), t2 = grp * W + (uint32_t)(e % W);
+            if constexpr (rt_tiled(LG1)) {  // pass B's tiles: side 0 of pair k1, or side 1 of N1 - k1
+                const uint32_t pr = k1 < (uint32_t)N1 / 2 ? k1 : (((uint32_t)N1 - k1) & ((uint32_t)N1 / 2 - 1));
+                const uint32_t sd = k1 < (uint32_t)N1 / 2 ? 0u : 1u;
+                v[i] = Rp[((size_t)(t2 / RT_COLS) * (N1 / 2) + pr) * (2 * RT_COLS) + sd * RT_COLS + t2 % RT_COLS];
+            } else {
+                v[i] = Rp[(size_t)k1 * g.N2 + t2];
+            }
+        }
     }
 #pragma unroll
     for (int i = 0; i < LPT; ++i) {

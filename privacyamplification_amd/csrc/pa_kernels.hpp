@@ -35,10 +35,11 @@ __host__ __device__ constexpr int cls_table_offset(int D) {
 }
 constexpr int CLS_TABLE_TOTAL = cls_table_offset(32);  // D = 1 .. 16
 
-// Columns per workgroup in pass A' (tile of W x N1 complex <= 16384 values in LDS; N1 = 8192 takes
-// 2 columns one at a time, two workgroups per compute unit) and its threads.
+// Columns per workgroup in pass A' (tile of W x N1 complex <= 16384 values in LDS) and its threads.
+// N1 = 2^10 .. 2^12 take 8192 / N1 columns (about 68 KB of LDS) and N1 = 8192 takes 2 columns one at
+// a time, so that two workgroups share a compute unit.
 __host__ __device__ constexpr int inv_cols_width(int lg1) {
-    return lg1 >= 14 ? 1 : (lg1 == 13 ? 2 : ((14 - lg1) >= 4 ? 16 : (1 << (14 - lg1))));
+    return lg1 >= 14 ? 1 : (lg1 == 13 ? 2 : (lg1 >= 10 ? (1 << (13 - lg1)) : ((14 - lg1) >= 4 ? 16 : (1 << (14 - lg1)))));
 }
 __host__ __device__ constexpr int clamp_threads(int t) { return t < 64 ? 64 : (t > 1024 ? 1024 : t); }
 __host__ __device__ constexpr int inv_cols_threads(int lg1) {
@@ -70,11 +71,11 @@ hipError_t launch_fwd_cols(const uint32_t* tb, float2* Y, const Geom& g, const T
 // ROWS_SEEDPREP: S <- split spectrum of the seed rows held in S (in place, N+1 values).
 // ROWS_STATIC:   R' from the key rows in Yk and the split seed spectrum S.
 // ROWS_DYN:      R' from the key rows in Yk and the seed rows (pass-A output) in S.
-// R' goes to Rt in tiles when g.lg1 == RT_LG1 (N complex values), else to Yk in place.
+// R' goes to Rt in tiles when rt_tiled(g.lg1) (N complex values), else to Yk in place.
 // Persistent: at most `cus` workgroups (one per compute unit) walk the row pairs.
 hipError_t launch_rows(int mode, float2* Yk, float2* S, float2* Rt, const Geom& g, const Tables& t, int cus,
                        hipStream_t s);
-// R: pass B's output (the tile buffer when g.lg1 == RT_LG1).
+// R: pass B's output (the tile buffer when rt_tiled(g.lg1)).
 hipError_t launch_inv_cols(const float2* R, uint8_t* nib, const Geom& g, const uint32_t* wk,
                            const uint32_t* ws, const Tables& t, float* margin, hipStream_t s);
 // zero_counts != null: the launch also clears those 2 * POP_SLOTS popcount partials.
