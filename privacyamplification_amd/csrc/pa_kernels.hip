@@ -512,7 +512,8 @@ __global__ __launch_bounds__(FWD_NT, 2) void k_fwd_cols9p(const uint32_t* __rest
 // so the real split, product and inverse split need no data exchange, and their results are
 // exactly the inputs of inverse stage 1.
 //   ROWS_SEEDPREP: seed rows (pass-A output) -> split seed spectrum S, in place (+ S[N] at index N).
-//   ROWS_STATIC:   key rows + resident S -> R'[k1][t2] = w_N^{t2 k1} FFT_N2(conj Q)[t2], in place.
+//   ROWS_STATIC:   key rows + resident S -> R'[k1][t2] = w_N^{t2 k1} FFT_N2(conj Q)[t2], in place,
+//                  or into pass A''s tiles (Rt) when rt_tiled(lg1).
 //   ROWS_DYN:      key rows + seed rows; S is formed first and held in registers.
 // Persistent: each workgroup walks the items blockIdx.x, +gridDim.x, ...; the global inputs of
 // the next transform are requested while the current one runs (register prefetch).
