@@ -189,7 +189,8 @@ typedef struct {
     uint32_t max_batch;        /* blocks per batch (default 256) */
     uint64_t hbm_budget_bytes; /* device memory for in-flight blocks, split evenly over the slots
                                   (default 16 GiB); a block larger than one slot's share fails
-                                  with PA_ERR_NOMEM */
+                                  with PA_ERR_NOMEM.  pa_serve_start reserves a quarter of each
+                                  share (at most 4 GiB, plus half again) up front */
     uint32_t batch_flags;      /* PA_BATCH_* for every batch (default 0) */
     uint32_t reserved;
 } pa_serve_config;

@@ -1821,6 +1821,16 @@ int pa_serve_start(pa_engine* e, const pa_serve_config* cfg) {
         if (rc != PA_OK) return rc;
         PA_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
     }
+    {
+        // Each slot's arena starts at a quarter of its share of the budget (at most 4 GiB, plus the
+        // usual half again), so that the first batches do not stop the stream to allocate; it
+        // still grows on demand up to the share.  Best effort: a failure leaves on-demand growth.
+        std::lock_guard<std::mutex> lk(e->mu);
+        const uint64_t share = sv->cfg.hbm_budget_bytes / sv->slots.size();
+        const uint64_t want = std::min<uint64_t>(share / 4, 4ull << 30);
+        for (auto& sl : sv->slots)
+            if (ensure_arena(e, sl.bs, want, share) != PA_OK) break;
+    }
     Server* raw = sv.release();
     e->server = raw;
     raw->th = std::thread(serve_loop, e, raw);
