@@ -318,6 +318,10 @@ __device__ __forceinline__ void buf_store_nt(float2 v, __amdgpu_buffer_rsrc_t rs
     const pa_u2v x = {__float_as_uint(v.x), __float_as_uint(v.y)};
     __builtin_amdgcn_raw_buffer_store_b64(x, rs, (int)voff, (int)soff, 2);  // aux 2: nt
 }
+__device__ __forceinline__ float2 buf_load_f2(__amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff) {
+    const pa_u2v x = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)voff, (int)soff, 0);
+    return make_float2(__uint_as_float(x.x), __uint_as_float(x.y));
+}
 __device__ __forceinline__ uint32_t buf_load_u32(__amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff) {
     return __builtin_amdgcn_raw_buffer_load_b32(rs, (int)voff, (int)soff, 0);
 }
@@ -565,11 +569,14 @@ __global__ __launch_bounds__(ROWS_NT) void k_rows(float2* __restrict__ Yk, float
     // stage-1 butterfly of this thread: row (t >> 8), j = t & 255, inputs j + 256 r
     const int s1seq = tid >> 8, s1j = tid & 255;
 
+    // Buffer loads: one 32-bit offset register per thread.  A 64-bit row pointer here was kept
+    // spilled, and its reload before the next item's request waited for the output stores.
     auto s1_load = [&](const float2* __restrict__ src, uint32_t w, float2 (&v)[32]) {
         const uint32_t row = s1seq ? ((w == 0) ? (N1 >> 1) : (N1 - w)) : w;
-        const float2* rp = src + (size_t)row * N2 + s1j;
+        const __amdgpu_buffer_rsrc_t rs = make_rsrc(src, N1 * N2 * 8u);
+        const uint32_t vo = (row * N2 + (uint32_t)s1j) * 8u;
 #pragma unroll
-        for (int r = 0; r < 32; ++r) v[r] = rp[256 * r];
+        for (int r = 0; r < 32; ++r) v[r] = buf_load_f2(rs, vo, 2048u * r);
     };
     // forward stages 1 (inputs in v) .. 3; after stage 1 has consumed v, refill(v) requests the
     // next transform's inputs into the same registers; ends with this thread's stage-3 outputs
