@@ -294,7 +294,8 @@ struct pa_engine {
     uint32_t stage_count[16] = {};
     // batched launches
     BatchSlot batch;                         // pa_batch_device / pa_run_batch
-    std::map<std::string, uint32_t> tuned;   // row-pass grid per (L, seed mode, count)
+    struct Tuned { uint32_t grid_x, fwd_per; };
+    std::map<std::string, Tuned> tuned;      // launch shape per (L, seed mode, count)
     struct Server* server = nullptr;         // pa_serve_*
     struct PinRange { uint64_t bytes; uint8_t* dev; };
     std::map<uintptr_t, PinRange> pinned;    // pa_host_alloc ranges: base -> size, mapped address
@@ -1015,6 +1016,7 @@ struct BatchGroup {
     uint32_t tab0 = 0;          // first table row of the group
     uint32_t tiles_y = 1;       // pack grid.y: the largest window of the group
     uint32_t grid_x = 0;        // row-pass workgroups per block
+    uint32_t fwd_per = 0;       // pass A walk length (0: the launcher's default)
     double cost = 0.0;
 };
 
@@ -1104,7 +1106,7 @@ int ensure_arena(pa_engine* e, BatchSlot& sl, uint64_t bytes, uint64_t limit = 0
 
 // All launches of one group on stream s.
 int batch_group_launch(pa_engine* e, const BatchPlan& p, const BatchGroup& gr, const pa::BlockArgs* tab,
-                       uint32_t grid_x, hipStream_t s) {
+                       uint32_t grid_x, uint32_t fwd_per, hipStream_t s) {
     const pa::BlockArgs* bt = tab + gr.tab0;
     const uint32_t cnt = (uint32_t)gr.idx.size();
     const Geom& g0 = p.items[gr.idx[0]].g;
@@ -1115,10 +1117,10 @@ int batch_group_launch(pa_engine* e, const BatchPlan& p, const BatchGroup& gr, c
     }
     PA_HIP(pa::launch_batch_zero(bt, cnt, s));
     PA_HIP(pa::launch_batch_transpose(bt, cnt, g0, 0, s));
-    PA_HIP(pa::launch_batch_fwd_cols(bt, cnt, g0, t, 0, s));
+    PA_HIP(pa::launch_batch_fwd_cols(bt, cnt, g0, t, 0, fwd_per, s));
     if (gr.dyn) {
         PA_HIP(pa::launch_batch_transpose(bt, cnt, g0, 1, s));
-        PA_HIP(pa::launch_batch_fwd_cols(bt, cnt, g0, t, 1, s));
+        PA_HIP(pa::launch_batch_fwd_cols(bt, cnt, g0, t, 1, fwd_per, s));
     }
     PA_HIP(pa::launch_batch_rows(gr.dyn ? pa::ROWS_DYN : pa::ROWS_STATIC, bt, cnt, g0, t, grid_x, s));
     PA_HIP(pa::launch_batch_inv_cols(bt, cnt, g0, t, s));
@@ -1129,7 +1131,7 @@ int batch_group_launch(pa_engine* e, const BatchPlan& p, const BatchGroup& gr, c
 // Groups on the fork streams (largest first onto the least-loaded stream), joined back into s.
 int batch_launch_all(pa_engine* e, BatchSlot& sl, const BatchPlan& p, const pa::BlockArgs* tab, hipStream_t s) {
     const size_t ng = p.groups.size();
-    if (ng == 1) return batch_group_launch(e, p, p.groups[0], tab, p.groups[0].grid_x, s);
+    if (ng == 1) return batch_group_launch(e, p, p.groups[0], tab, p.groups[0].grid_x, p.groups[0].fwd_per, s);
     std::vector<size_t> order(ng);
     for (size_t i = 0; i < ng; ++i) order[i] = i;
     std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return p.groups[a].cost > p.groups[b].cost; });
@@ -1145,7 +1147,7 @@ int batch_launch_all(pa_engine* e, BatchSlot& sl, const BatchPlan& p, const pa::
             used[k] = true;
         }
         load[k] += p.groups[gi].cost;
-        int rc = batch_group_launch(e, p, p.groups[gi], tab, p.groups[gi].grid_x, sl.fork[k]);
+        int rc = batch_group_launch(e, p, p.groups[gi], tab, p.groups[gi].grid_x, p.groups[gi].fwd_per, sl.fork[k]);
         if (rc != PA_OK) return rc;
     }
     for (int k = 0; k < BatchSlot::FORKS; ++k) {
@@ -1160,37 +1162,61 @@ std::string tune_key(const BatchGroup& gr) {
     return std::to_string(gr.L) + (gr.dyn ? "d" : "s") + std::to_string(gr.idx.size());
 }
 
-// Times the row-pass grid candidates of a group (whole group sequence per candidate; the outputs
-// are rewritten by the real run that follows) and records the fastest.
+// The autotuner: times a group's whole launch sequence (the outputs are rewritten by the real run
+// that follows) over its launch-shape knobs and records the fastest shape per (L, seed mode,
+// count).  Knobs, tuned one after the other: the pass A walk length (column groups per persistent
+// workgroup, for N1 = 2^9 .. 2^13), then the row-pass grid (workgroups per block).
 int batch_tune_group(pa_engine* e, const BatchPlan& p, BatchGroup& gr, const pa::BlockArgs* tab, hipStream_t s) {
-    const uint32_t items = p.items[gr.idx[0]].g.N1 / 2u;
+    const Geom& g0 = p.items[gr.idx[0]].g;
+    const uint32_t items = g0.N1 / 2u;
     const uint32_t cnt = (uint32_t)gr.idx.size();
     const uint32_t base = std::max<uint32_t>(1u, ((uint32_t)e->cus + cnt - 1u) / cnt);
     hipEvent_t a, b;
     PA_HIP(hipEventCreate(&a));
     PA_HIP(hipEventCreate(&b));
-    float best = 1e30f;
-    uint32_t best_x = std::min(base, items);
-    for (uint32_t x = 1; x <= items; x *= 2) {
-        if (x * 4u < base || x > base * 8u) continue;
-        float ms = 0.f;
-        for (int rep = 0; rep < 2; ++rep) {  // first run warms up
+    auto time_shape = [&](uint32_t x, uint32_t per, float* ms) -> int {
+        for (int rep = 0; rep < 2; ++rep) {  // the first run warms up
             PA_HIP(hipEventRecord(a, s));
-            int rc = batch_group_launch(e, p, gr, tab, x, s);
+            int rc = batch_group_launch(e, p, gr, tab, x, per, s);
             if (rc != PA_OK) return rc;
             PA_HIP(hipEventRecord(b, s));
             PA_HIP(hipEventSynchronize(b));
-            PA_HIP(hipEventElapsedTime(&ms, a, b));
+            PA_HIP(hipEventElapsedTime(ms, a, b));
         }
-        if (ms < best) {
+        return PA_OK;
+    };
+    int rc = PA_OK;
+    uint32_t best_x = std::min(base, items), best_per = 0;
+    if (g0.lg1 >= 9 && g0.lg1 <= 13) {
+        // walk lengths that leave at least one workgroup per compute unit (and the default)
+        const uint32_t wgs = (g0.N2 / 16u) * g0.D * cnt;
+        float best = 1e30f;
+        for (uint32_t per = 1; per <= pa::FWD_PER_MAX && rc == PA_OK; per *= 2) {
+            if (per > 1u && wgs / per < (uint32_t)e->cus) break;
+            float ms = 0.f;
+            rc = time_shape(best_x, per, &ms);
+            if (rc == PA_OK && ms < best) {
+                best = ms;
+                best_per = per;
+            }
+        }
+    }
+    float best = 1e30f;
+    for (uint32_t x = 1; x <= items && rc == PA_OK; x *= 2) {
+        if (x * 4u < base || x > base * 8u) continue;
+        float ms = 0.f;
+        rc = time_shape(x, best_per, &ms);
+        if (rc == PA_OK && ms < best) {
             best = ms;
             best_x = x;
         }
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
-    e->tuned[tune_key(gr)] = best_x;
+    if (rc != PA_OK) return rc;
+    e->tuned[tune_key(gr)] = {best_x, best_per};
     gr.grid_x = best_x;
+    gr.fwd_per = best_per;
     return PA_OK;
 }
 
@@ -1254,8 +1280,9 @@ int batch_execute(pa_engine* e, BatchSlot& sl, BatchPlan& p, const pa_block* dev
             auto t = e->tuned.find(tune_key(gr));
             const uint32_t items = p.items[gr.idx[0]].g.N1 / 2u;
             const uint32_t cnt = (uint32_t)gr.idx.size();
-            gr.grid_x = t != e->tuned.end() ? t->second
+            gr.grid_x = t != e->tuned.end() ? t->second.grid_x
                                             : std::min(items, std::max<uint32_t>(1u, ((uint32_t)e->cus + cnt - 1u) / cnt));
+            gr.fwd_per = t != e->tuned.end() ? t->second.fwd_per : 0u;
         }
     }
     // the table and the input ranges in one copy launch (read from mapped host memory)
@@ -1300,7 +1327,8 @@ int batch_execute(pa_engine* e, BatchSlot& sl, BatchPlan& p, const pa_block* dev
     auto replay = [&]() -> int {
         std::string key = std::to_string((uintptr_t)sl.arena) + ":";
         for (auto& gr : p.groups)
-            key += tune_key(gr) + "x" + std::to_string(gr.grid_x) + "y" + std::to_string(gr.tiles_y) + ";";
+            key += tune_key(gr) + "x" + std::to_string(gr.grid_x) + "p" + std::to_string(gr.fwd_per) + "y" +
+                   std::to_string(gr.tiles_y) + ";";
         auto f = sl.graphs.find(key);
         if (f == sl.graphs.end()) {
             hipGraph_t graph = nullptr;

@@ -1422,14 +1422,16 @@ hipError_t batch_transpose_t(const BlockArgs* bt, uint32_t count, const Geom& g0
 }
 template <int LGM, int D>
 hipError_t batch_fwd_cols_t(const BlockArgs* bt, uint32_t count, const Geom& g0, const Tables& t, int is_seed,
-                            hipStream_t s) {
+                            uint32_t per, hipStream_t s) {
     if constexpr (LGM == 9) {
-        // one class per workgroup walking `per` column groups: 16, fewer while the launch would
-        // have under 512 workgroups (on the 256-block mix, 16 runs 2 % faster than 4 and 5 % faster
-        // than 1)
+        // one class per workgroup walking `per` column groups (a power of two dividing the N2 / 16
+        // groups); by default 16, fewer while the launch would have under 512 workgroups
         const uint32_t groups = g0.N2 / 16u;
-        uint32_t per = 16u;
-        while (per > 1u && groups * (uint32_t)D / per * count < 512u) per >>= 1;
+        if (per == 0u) {
+            per = 16u;
+            while (per > 1u && groups * (uint32_t)D / per * count < 512u) per >>= 1;
+        }
+        if (per > FWD_PER_MAX || (per & (per - 1u)) != 0u || groups % per != 0u) return hipErrorInvalidValue;
         const uint32_t G = groups * (uint32_t)D / per;
         k_fwd_cols9p<D, true><<<dim3(G, 1, count), FWD_NT, 0, s>>>(nullptr, nullptr, g0, t.tw14, t.lo, t.hi, t.cls,
                                                                   nullptr, nullptr, per, bt, is_seed);
@@ -1465,23 +1467,23 @@ hipError_t launch_batch_transpose(const BlockArgs* bt, uint32_t count, const Geo
 }
 
 hipError_t launch_batch_fwd_cols(const BlockArgs* bt, uint32_t count, const Geom& g0, const Tables& t, int is_seed,
-                                 hipStream_t s) {
+                                 uint32_t per, hipStream_t s) {
     switch (g0.lg1) {
-    case 1: return batch_fwd_cols_t<1, 1>(bt, count, g0, t, is_seed, s);
-    case 2: return batch_fwd_cols_t<2, 1>(bt, count, g0, t, is_seed, s);
-    case 3: return batch_fwd_cols_t<3, 1>(bt, count, g0, t, is_seed, s);
-    case 4: return batch_fwd_cols_t<4, 1>(bt, count, g0, t, is_seed, s);
-    case 5: return batch_fwd_cols_t<5, 1>(bt, count, g0, t, is_seed, s);
-    case 6: return batch_fwd_cols_t<6, 1>(bt, count, g0, t, is_seed, s);
-    case 7: return batch_fwd_cols_t<7, 1>(bt, count, g0, t, is_seed, s);
-    case 8: return batch_fwd_cols_t<8, 1>(bt, count, g0, t, is_seed, s);
-    case 9: return batch_fwd_cols_t<9, 1>(bt, count, g0, t, is_seed, s);
-    case 10: return batch_fwd_cols_t<9, 2>(bt, count, g0, t, is_seed, s);
-    case 11: return batch_fwd_cols_t<9, 4>(bt, count, g0, t, is_seed, s);
-    case 12: return batch_fwd_cols_t<9, 8>(bt, count, g0, t, is_seed, s);
-    case 13: return batch_fwd_cols_t<9, 16>(bt, count, g0, t, is_seed, s);
-    case 14: return batch_fwd_cols_t<10, 16>(bt, count, g0, t, is_seed, s);
-    case 15: return batch_fwd_cols_t<11, 16>(bt, count, g0, t, is_seed, s);
+    case 1: return batch_fwd_cols_t<1, 1>(bt, count, g0, t, is_seed, per, s);
+    case 2: return batch_fwd_cols_t<2, 1>(bt, count, g0, t, is_seed, per, s);
+    case 3: return batch_fwd_cols_t<3, 1>(bt, count, g0, t, is_seed, per, s);
+    case 4: return batch_fwd_cols_t<4, 1>(bt, count, g0, t, is_seed, per, s);
+    case 5: return batch_fwd_cols_t<5, 1>(bt, count, g0, t, is_seed, per, s);
+    case 6: return batch_fwd_cols_t<6, 1>(bt, count, g0, t, is_seed, per, s);
+    case 7: return batch_fwd_cols_t<7, 1>(bt, count, g0, t, is_seed, per, s);
+    case 8: return batch_fwd_cols_t<8, 1>(bt, count, g0, t, is_seed, per, s);
+    case 9: return batch_fwd_cols_t<9, 1>(bt, count, g0, t, is_seed, per, s);
+    case 10: return batch_fwd_cols_t<9, 2>(bt, count, g0, t, is_seed, per, s);
+    case 11: return batch_fwd_cols_t<9, 4>(bt, count, g0, t, is_seed, per, s);
+    case 12: return batch_fwd_cols_t<9, 8>(bt, count, g0, t, is_seed, per, s);
+    case 13: return batch_fwd_cols_t<9, 16>(bt, count, g0, t, is_seed, per, s);
+    case 14: return batch_fwd_cols_t<10, 16>(bt, count, g0, t, is_seed, per, s);
+    case 15: return batch_fwd_cols_t<11, 16>(bt, count, g0, t, is_seed, per, s);
     default: return hipErrorInvalidValue;
     }
 }

@@ -90,8 +90,11 @@ hipError_t launch_small(const uint32_t* key, const uint32_t* seed, const float2*
 uint32_t pack_tiles(const Geom& g);  // grid.y the pack stage needs for this block's window
 hipError_t launch_batch_zero(const BlockArgs* bt, uint32_t count, hipStream_t s);
 hipError_t launch_batch_transpose(const BlockArgs* bt, uint32_t count, const Geom& g0, int is_seed, hipStream_t s);
+// per: column groups each workgroup of the persistent pass A (N1 = 2^9 .. 2^13) walks, a power of
+// two up to 32; 0 picks 16, halved while the launch would have under 512 workgroups.
 hipError_t launch_batch_fwd_cols(const BlockArgs* bt, uint32_t count, const Geom& g0, const Tables& t, int is_seed,
-                                 hipStream_t s);
+                                 uint32_t per, hipStream_t s);
+constexpr uint32_t FWD_PER_MAX = 32;
 // grid_x workgroups per block walk that block's row pairs (0 = one per row pair).
 hipError_t launch_batch_rows(int mode, const BlockArgs* bt, uint32_t count, const Geom& g0, const Tables& t,
                              uint32_t grid_x, hipStream_t s);
