@@ -139,8 +139,8 @@ typedef struct {
 } pa_block;
 
 /* Batch options: PA_BATCH_GRAPH replays a captured hipGraph for a batch of the same shape
- * (lengths classes, seed modes and counts); PA_BATCH_TUNE times the row-pass grid candidates of a
- * shape not seen before and keeps the fastest. */
+ * (lengths classes, seed modes and counts); PA_BATCH_TUNE times the launch shapes of a shape not
+ * seen before (pass A walk length, then row-pass grid) and keeps the fastest. */
 enum pa_batch_flags {
     PA_BATCH_GRAPH = 1u,
     PA_BATCH_TUNE = 2u
