@@ -549,7 +549,7 @@ def main():
     if args.workload == "precision":
         return bench_precision(args)
     man = manifest()
-    ctx = Ctx()
+    ctx = Ctx(max_n_log2=max(27, args.log2n))
     if args.copy_calibration:
         # a plain streaming kernel of known traffic: reads 512 MiB, writes 512 MiB per launch
         src = torch.empty(1 << 27, dtype=torch.float32, device=ctx.dev).fill_(1.0)

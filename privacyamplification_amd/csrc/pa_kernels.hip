@@ -220,7 +220,7 @@ __global__ __launch_bounds__(TB_NT) void k_transpose_bits(const uint32_t* __rest
 // class values go to LDS and the M-point FFT runs there.  M = 512 (N1 = 512 D, the 2^27-bit
 // block's shape) runs in k_fwd_cols9p instead.
 template <int LGM, int D, bool BATCH, int CPW = fwd_cols_width(LGM)>
-__global__ __launch_bounds__(FWD_NT) void k_fwd_cols(const uint32_t* __restrict__ tb,
+__global__ __launch_bounds__(fwd_cols_threads(LGM)) void k_fwd_cols(const uint32_t* __restrict__ tb,
                                                      float2* __restrict__ Y, Geom g,
                                                      const float2* __restrict__ tw14,
                                                      const float2* __restrict__ twlo,
@@ -241,7 +241,7 @@ __global__ __launch_bounds__(FWD_NT) void k_fwd_cols(const uint32_t* __restrict_
         Y = is_seed ? a.Ys : a.Yk;
     }
     constexpr int M = 1 << LGM;
-    constexpr int NT = FWD_NT;
+    constexpr int NT = fwd_cols_threads(LGM);
     constexpr int SEQS = seq_stride(M);
     static_assert(LGM != 9, "512-point classes run in k_fwd_cols9p");
     __shared__ float2 buf[CPW * SEQS];
@@ -300,11 +300,25 @@ __global__ __launch_bounds__(FWD_NT) void k_fwd_cols(const uint32_t* __restrict_
     }
     __syncthreads();
     lds_fft<LGM, CPW, SEQS, NT>(buf, twl);
-    for (int e = threadIdx.x; e < CPW * M; e += NT) {
-        const int col = e % CPW, k1p = e / CPW;
-        const uint32_t k1 = c + (uint32_t)D * (uint32_t)k1p, t2 = grp * CPW + (uint32_t)col;
-        const float2 tw = tw_two_level(twlo, twhi, (uint32_t)((2ull * t2 * k1) & (g.L - 1u)));
-        st_nt(&Y[(size_t)k1 * g.N2 + t2], cmul(buf[col * SEQS + padi(k1p)], tw));
+    // Output e = tid + NT i is column tid mod CPW, k1' = tid / CPW + RPI i: its twiddle
+    // w_L^{2 t2 k1} is base step^i, re-anchored from the table every 8 steps (a gather per point
+    // cost more than the transform at N1 >= 2^14)
+    static_assert(NT % CPW == 0, "a thread keeps one column");
+    constexpr int RPI = NT / CPW;
+    const int col = threadIdx.x % CPW, jt = threadIdx.x / CPW;
+    const uint32_t t2 = grp * CPW + (uint32_t)col;
+    const uint64_t L1m = g.L - 1u;
+    auto tw_at = [&](uint32_t k1p) {
+        return tw_two_level(twlo, twhi, (uint32_t)((2ull * t2 * (c + (uint64_t)D * k1p)) & L1m));
+    };
+    const float2 step = tw_two_level(twlo, twhi, (uint32_t)((2ull * t2 * (uint64_t)D * RPI) & L1m));
+    float2 tw = make_float2(1.f, 0.f);
+    for (int i = 0; i < (CPW * M + NT - 1) / NT; ++i) {
+        const uint32_t k1p = (uint32_t)jt + (uint32_t)RPI * (uint32_t)i;
+        if (k1p >= (uint32_t)M) break;  // CPW M < NT: the upper threads have no output
+        tw = (i & 7) == 0 ? tw_at(k1p) : cmul(tw, step);
+        const uint32_t k1 = c + (uint32_t)D * k1p;
+        st_nt(&Y[(size_t)k1 * g.N2 + t2], cmul(buf[col * SEQS + padi((int)k1p)], tw));
     }
 }
 
@@ -1255,7 +1269,7 @@ hipError_t launch_fwd_cols_t(const uint32_t* tb, float2* Y, const Geom& g, const
                                                                      per, nullptr, 0);
     } else {
         const uint32_t grid = (g.N2 / (uint32_t)fwd_cols_width(LGM)) * (uint32_t)D;
-        k_fwd_cols<LGM, D, false><<<dim3(grid, Y2 ? 2 : 1), FWD_NT, 0, s>>>(tb, Y, g, t.tw14, t.lo, t.hi, t.cls,
+        k_fwd_cols<LGM, D, false><<<dim3(grid, Y2 ? 2 : 1), fwd_cols_threads(LGM), 0, s>>>(tb, Y, g, t.tw14, t.lo, t.hi, t.cls,
                                                                         nullptr, 0, tb2, Y2);
     }
     return hipGetLastError();
@@ -1436,7 +1450,8 @@ hipError_t batch_fwd_cols_t(const BlockArgs* bt, uint32_t count, const Geom& g0,
         k_fwd_cols9p<D, true><<<dim3(G, 1, count), FWD_NT, 0, s>>>(nullptr, nullptr, g0, t.tw14, t.lo, t.hi, t.cls,
                                                                   nullptr, nullptr, per, bt, is_seed);
     } else {
-        k_fwd_cols<LGM, D, true><<<dim3((g0.N2 / (uint32_t)fwd_cols_width(LGM)) * (uint32_t)D, 1, count), FWD_NT, 0,
+        k_fwd_cols<LGM, D, true><<<dim3((g0.N2 / (uint32_t)fwd_cols_width(LGM)) * (uint32_t)D, 1, count),
+                                   fwd_cols_threads(LGM), 0,
                                    s>>>(nullptr, nullptr, g0, t.tw14, t.lo, t.hi, t.cls, bt, is_seed, nullptr, nullptr);
     }
     return hipGetLastError();

@@ -47,6 +47,9 @@ __host__ __device__ constexpr int inv_cols_threads(int lg1) {
 }
 // Columns per workgroup in pass A (16; 8 when the class length M = 2048 would overflow LDS).
 __host__ __device__ constexpr int fwd_cols_width(int lgm) { return lgm >= 11 ? 8 : 16; }
+// Threads of the generic pass A: 1024 for classes of 1024 and 2048 points, whose 128 KB of columns
+// leave room for one workgroup per CU, so that it brings 16 waves instead of 4.
+__host__ __device__ constexpr int fwd_cols_threads(int lgm) { return lgm >= 10 ? 1024 : FWD_NT; }
 // Bytes of the pass-A' output: nibbles per column pair (N/2), or 2-bit crumbs per column (N) when
 // a workgroup holds a single column.
 __host__ __device__ constexpr uint64_t nib_bytes(uint32_t N, uint32_t W) { return (W == 1 ? (uint64_t)N : N / 2) + 64; }
