@@ -24,17 +24,18 @@ constexpr int ROW_LOG = 13;            // N2 = 8192 for the four-step path
 constexpr int SMALL_MAX_LOG_N = 13;    // N <= 8192 -> single-workgroup path
 constexpr int MAX_L_LOG = 29;          // largest transform served (L = 2^29; the reference stops at 2^27)
 
-// Pass B's output R'[k1][t2] when 2^RT_LG1_MIN <= N1 <= 2^RT_LG1 (transforms of 2^24 to 2^27 bits)
-// goes to its own buffer in tiles: row pair p (rows p and N1-p, or 0 and N1/2 for p = 0), side s
-// (0: row p, 1: its partner) at Rt[((t2 / RT_COLS) N1/2 + p) 2 RT_COLS + s RT_COLS + t2 mod RT_COLS].
+// Pass B's output R'[k1][t2] when 2^RT_LG1_MIN <= N1 <= 2^RT_LG1_MAX (transforms of 2^24 to 2^29
+// bits) goes to its own buffer in tiles: row pair p (rows p and N1-p, or 0 and N1/2 for p = 0), side
+// s (0: row p, 1: its partner) at Rt[((t2 / RT_COLS) N1/2 + p) 2 RT_COLS + s RT_COLS + t2 mod RT_COLS].
 // A tile holds both rows' RT_COLS columns (2 x 128 bytes): pass B writes whole 128-byte lines, and a
-// workgroup of pass A' (2 to 8 columns) reads its columns from consecutive lines along p instead of
-// from rows 8 N2 bytes apart.
+// workgroup of pass A' (1 to 8 columns) reads its columns from consecutive lines along p instead of
+// from rows 8 N2 bytes apart.  At N1 = 2^RT_LG1 pass A' runs two 256-thread workgroups per CU.
 constexpr int RT_LG1_MIN = 10;
 constexpr int RT_LG1 = 13;
+constexpr int RT_LG1_MAX = 15;
 constexpr int RT_COLS = 16;
 __host__ __device__ constexpr bool rt_tiled(uint32_t lg1) {
-    return lg1 >= (uint32_t)RT_LG1_MIN && lg1 <= (uint32_t)RT_LG1;
+    return lg1 >= (uint32_t)RT_LG1_MIN && lg1 <= (uint32_t)RT_LG1_MAX;
 }
 
 struct Geom {

@@ -786,7 +786,7 @@ __global__ __launch_bounds__(ROWS_NT) void k_rows(float2* __restrict__ Yk, float
 // Writes one nibble (4 conv bits, MSB first) per (column pair, t1) inside the output window, in
 // column-pair-major order nibT[pair * N1 + t1] so that each workgroup's writes are contiguous.
 template <int LG1, bool BATCH>
-__global__ __launch_bounds__(inv_cols_threads(LG1), rt_tiled(LG1) ? 2 : 1) void k_inv_cols(
+__global__ __launch_bounds__(inv_cols_threads(LG1), rt_tiled(LG1) && LG1 <= RT_LG1 ? 2 : 1) void k_inv_cols(
     const float2* __restrict__ Rp, uint8_t* __restrict__ nibT, Geom g,
     const uint32_t* __restrict__ wk_cnt, const uint32_t* __restrict__ ws_cnt,
     const float2* __restrict__ tw14, const float2* __restrict__ twlo, const float2* __restrict__ twhi,
@@ -805,12 +805,72 @@ __global__ __launch_bounds__(inv_cols_threads(LG1), rt_tiled(LG1) ? 2 : 1) void 
     constexpr int NT = inv_cols_threads(LG1);
     constexpr int LGL = LG1 < 14 ? LG1 : 14;  // LDS transform length (N1 = 2^15: two halves)
     constexpr int SEQS = seq_stride(1 << LGL);
-    constexpr int WL = LG1 == RT_LG1 ? 1 : W;  // columns held in LDS at once
+    constexpr int WL = (LG1 == RT_LG1 || LG1 >= 14) ? 1 : W;  // columns held in LDS at once
     __shared__ float2 buf[WL * SEQS];
     __shared__ float2 twl[TWL_SIZE];
     load_twl(twl, tw14, NT);
     const uint32_t grp = xcd_logical(blockIdx.x, gridDim.x);
-    if constexpr (LG1 >= 14) {
+    if constexpr (LG1 == 14) {
+        // Column pair t2 = 2 grp, 2 grp + 1 per 1024-thread workgroup, one column in LDS at a time
+        // (one workgroup per CU).  Thread j loads both columns of row k1 = j + 1024 r from pass B's
+        // tiles with one 16-byte load (side 0 of pair k1 for k1 < N1/2, side 1 of pair N1 - k1
+        // after).  The first column's bits wait packed in a register (2 per t1 = j + 1024 q) for the
+        // second's; the output is one nibble per (column pair, t1).
+        static_assert(W == 2 && NT == 1024 && rt_tiled(LG1), "paired column pass expects 2 columns, 1024 threads");
+        constexpr int PT = N1 / NT;
+        const int j = threadIdx.x;
+        float2 v0[PT], v1[PT];
+        const uint32_t c0 = grp * W;
+        {
+            const float2* tp = Rp + (size_t)(c0 / RT_COLS) * (N1 / 2) * (2 * RT_COLS) + (c0 % RT_COLS);
+#pragma unroll
+            for (int r = 0; r < PT; ++r) {
+                const uint32_t k1 = (uint32_t)j + (uint32_t)NT * r;
+                const uint32_t pr = r < PT / 2 ? k1 : (((uint32_t)N1 - k1) & ((uint32_t)N1 / 2 - 1));
+                const float4 x = *reinterpret_cast<const float4*>(tp + (size_t)pr * (2 * RT_COLS) + (r < PT / 2 ? 0 : RT_COLS));
+                v0[r] = make_float2(x.x, x.y);
+                v1[r] = make_float2(x.z, x.w);
+            }
+        }
+        const float corr = correction(pop_total(wk_cnt), pop_total(ws_cnt), g.L);
+        const float invN = g.inv_N;
+        float dist = 0.f;
+        auto inwin = [&](uint32_t t1) {
+            const uint32_t i0 = 2u * (c0 + g.N2 * t1);
+            return !(i0 + 4u <= g.out_off || i0 >= g.out_off + g.out_bits);
+        };
+        __syncthreads();  // twl
+#pragma unroll
+        for (int r = 0; r < PT; ++r) buf[padi(j + NT * r)] = v0[r];
+        __syncthreads();
+        lds_fft<LG1, 1, SEQS, NT>(buf, twl);
+        uint32_t lo = 0u;
+#pragma unroll
+        for (int q = 0; q < PT; ++q) {
+            const uint32_t t1 = (uint32_t)j + (uint32_t)NT * q;
+            if (!inwin(t1)) continue;
+            const float2 G = buf[padi((int)t1)];
+            const uint32_t b0 = round_bit(G.x * invN + corr, dist);
+            const uint32_t b1 = round_bit(-G.y * invN + corr, dist);
+            lo |= ((b0 << 1) | b1) << (2 * q);
+        }
+        __syncthreads();  // the first column's reads are done
+#pragma unroll
+        for (int r = 0; r < PT; ++r) buf[padi(j + NT * r)] = v1[r];
+        __syncthreads();
+        lds_fft<LG1, 1, SEQS, NT>(buf, twl);
+#pragma unroll
+        for (int q = 0; q < PT; ++q) {
+            const uint32_t t1 = (uint32_t)j + (uint32_t)NT * q;
+            if (!inwin(t1)) continue;
+            const float2 G = buf[padi((int)t1)];
+            const uint32_t b2 = round_bit(G.x * invN + corr, dist);
+            const uint32_t b3 = round_bit(-G.y * invN + corr, dist);
+            st_nt_u8(&nibT[(size_t)grp * N1 + t1], (uint8_t)((((lo >> (2 * q)) & 3u) << 2) | (b2 << 1) | b3));
+        }
+        if (margin) margin_publish(dist, margin);
+        return;
+    } else if constexpr (LG1 >= 14) {
         // One column t2 = grp per workgroup; output one 2-bit crumb per t1 at nibT[t2 * N1 + t1]
         // (bits c[2t], c[2t+1] of t = t2 + N2 t1).  N1 = 2^15 splits the column first by one
         // radix-2 step in registers (X[2k] = FFT(a)[k], X[2k+1] = FFT(b)[k] with a = x[j] + x[j+N1/2],
@@ -823,8 +883,20 @@ __global__ __launch_bounds__(inv_cols_threads(LG1), rt_tiled(LG1) ? 2 : 1) void 
         const float invN = g.inv_N;
         float dist = 0.f;
         float2 v[PT];
+        if constexpr (rt_tiled(LG1)) {
+            // pass B's tiles: rows k1 < N1/2 are side 0 of pair k1, the others side 1 of pair
+            // (N1 - k1) mod N1/2, so consecutive threads read consecutive tile lines
+            const float2* tp = Rp + (size_t)(t2 / RT_COLS) * (N1 / 2) * (2 * RT_COLS) + (t2 % RT_COLS);
 #pragma unroll
-        for (int r = 0; r < PT; ++r) v[r] = Rp[(size_t)(threadIdx.x + NT * r) * g.N2 + t2];
+            for (int r = 0; r < PT; ++r) {
+                const uint32_t k1 = threadIdx.x + (uint32_t)NT * r;
+                const uint32_t pr = r < PT / 2 ? k1 : (((uint32_t)N1 - k1) & ((uint32_t)N1 / 2 - 1));
+                v[r] = tp[(size_t)pr * (2 * RT_COLS) + (r < PT / 2 ? 0 : RT_COLS)];
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < PT; ++r) v[r] = Rp[(size_t)(threadIdx.x + NT * r) * g.N2 + t2];
+        }
         __syncthreads();  // twl
         constexpr int HALVES = N1 / H;
         if constexpr (HALVES == 2) {

@@ -37,9 +37,10 @@ constexpr int CLS_TABLE_TOTAL = cls_table_offset(32);  // D = 1 .. 16
 
 // Columns per workgroup in pass A' (tile of W x N1 complex <= 16384 values in LDS) and its threads.
 // N1 = 2^10 .. 2^12 take 8192 / N1 columns (about 68 KB of LDS) and N1 = 8192 takes 2 columns one at
-// a time, so that two workgroups share a compute unit.
+// a time, so that two workgroups share a compute unit.  N1 = 2^14 takes 2 columns one at a time
+// (one workgroup per CU), N1 = 2^15 one column in two LDS halves.
 __host__ __device__ constexpr int inv_cols_width(int lg1) {
-    return lg1 >= 14 ? 1 : (lg1 == 13 ? 2 : (lg1 >= 10 ? (1 << (13 - lg1)) : ((14 - lg1) >= 4 ? 16 : (1 << (14 - lg1)))));
+    return lg1 >= 15 ? 1 : (lg1 >= 13 ? 2 : (lg1 >= 10 ? (1 << (13 - lg1)) : ((14 - lg1) >= 4 ? 16 : (1 << (14 - lg1)))));
 }
 __host__ __device__ constexpr int clamp_threads(int t) { return t < 64 ? 64 : (t > 1024 ? 1024 : t); }
 __host__ __device__ constexpr int inv_cols_threads(int lg1) {

@@ -58,7 +58,8 @@ def test_workspace_sizes_and_rejections():
     assert P.workspace_bytes(1024, 1024) == 0        # m >= n
     assert P.workspace_bytes(3 << 20, 3 << 18, P.XOR_KEY_REST) == 0  # non-2^k without compression
     assert P.workspace_bytes(1 << 30, 3 << 27) == 0  # beyond the largest served transform (2^29)
-    assert 4 << 30 < P.workspace_bytes(1 << 29, 3 << 26) < 6 << 30
+    # at 2^29: key and seed spectra plus pass B's tiles, 2 GiB each
+    assert 6 << 30 < P.workspace_bytes(1 << 29, 3 << 26) < 7 << 30
 
 
 def test_no_cpu_fallback_when_library_missing(tmp_path, monkeypatch):
