@@ -12,7 +12,7 @@
 //   pass B  (rows)      row FFTs of length N2 on rows k1 and N1-k1, real split, DC zeroing, complex
 //                       product, inverse split, inverse row FFTs;
 //   pass A' (inv_cols)  inverse column FFTs of length N1, W columns per workgroup, round-mod-2 epilogue
-//                       (N1 >= 2^14: one column per workgroup, 2-bit crumbs instead of nibbles).
+//                       (N1 = 2^15: one column per workgroup, 2-bit crumbs instead of nibbles).
 // N <= 8192 (L <= 2^14) runs fully on chip in one workgroup (small path).
 #pragma once
 #include <hip/hip_runtime.h>
