@@ -321,6 +321,24 @@ def test_golden_manifest_batched(engine, oracle):
             engine.seed_drop(sid)
 
 
+def test_golden_manifest_batched_large(big_engine, oracle):
+    """The batched kernels beyond the reference's largest size: a 2^28 and a 2^29 manifest block in
+    one batch (pass B's tiles and the paired-column pass A' at N1 = 2^14, the single-column one at
+    2^15), against the committed digests."""
+    items = [it for it in _manifest() if it["name"] in ("c3_n28_p0.5", "c3_n29_p0.9")]
+    assert len(items) == 2
+    say("manifest 2^28 + 2^29 blocks in one batch")
+    blocks = []
+    for it in items:
+        n, m = it["n"], it["m"]
+        key = oracle.synth_bits(it["key_seed"], n + 1, it["p_key"])
+        seed = oracle.synth_bits(it["toeplitz_seed"], n)
+        blocks.append({"n": n, "m": m, "key": key, "seed": seed})
+    outs = big_engine.run_batch(blocks)
+    for it, o in zip(items, outs):
+        assert sha3(o) == it["sha3"], it["name"]
+
+
 def test_linearity_at_full_size(engine, oracle):
     """amp_out is GF(2)-linear in the key (T x and the key rest both are): out(a^b) = out(a)^out(b)."""
     n, m = 1 << 27, 3 << 24
